@@ -1,0 +1,255 @@
+"""Benchmark of the MI355X syclfft provider on one GPU.
+
+Default workload (the mixed-radix + Bluestein round-trip configuration): one step is
+forward + backward with orthogonal normalization of
+    complex<double> N=2520 (2^3 3^2 5 7) batch 1024,
+    complex<double> N=6561 (3^8)         batch 512,
+    complex<float>  N=10007 (prime, Bluestein) batch 128,
+six plan executions on device-resident inputs (uniform[-1,1) re/im, seeded).
+Metric: C2C GFLOP/s = sum 5 N log2(N) B / time, plus % of HBM peak from algorithmic
+bytes (one read + one write of every transform per execution).
+
+Also reported: the dominant kernel's roofline (HIP events around its launches on its
+stream), an fp32 batched sweep N = 2^12..2^20 at 512 MiB per buffer (HBM roofline
+fraction per N), rel-L2 of each sub-workload against the FP64 oracle on sampled lines, and
+the reference-algorithm CPU baseline (the oracle's restatement of the portable provider,
+OpenMP on the host cores) on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sycl-fft_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# (name, N, batch, double)
+WORKLOAD = [
+    ("c2_f64_2520x1024", 2520, 1024, True),
+    ("c2_f64_6561x512", 6561, 512, True),
+    ("c2_f32_10007x128", 10007, 128, False),
+]
+
+
+def flops(n, b):
+    return 5.0 * n * np.log2(n) * b
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def oracle():
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib.oracle_fftn_ld.argtypes = [ctypes.c_int, u64p, ctypes.c_uint64, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    for f in (lib.ref_portable_exec_f32, lib.ref_portable_exec_f64):
+        f.argtypes = [ctypes.c_int, u64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_max_threads.restype = ctypes.c_int
+    return lib
+
+
+def oracle_fft(lib, x, n, b, sign, norm):
+    xin = np.ascontiguousarray(x, np.complex128).reshape(-1)
+    out = np.empty_like(xin)
+    lens = (ctypes.c_uint64 * 3)(n, 1, 1)
+    lib.oracle_fftn_ld(1, lens, b, xin.ctypes.data, out.ctypes.data, sign, norm)
+    return out
+
+
+def cpu_baseline(lib, sample_scale=16, target_s=12.0):
+    """Reference portable-provider algorithm (restated, OpenMP) on 1/sample_scale of the
+    workload's lines, repeated until ~target_s seconds; returns GFLOP/s."""
+    total_flops = 0.0
+    elapsed = 0.0
+    reps = 0
+    g = np.random.default_rng(7)
+    cases = []
+    for name, n, b, dbl in WORKLOAD:
+        bs = max(1, b // sample_scale)
+        dt = np.complex128 if dbl else np.complex64
+        x = (g.uniform(-1, 1, n * bs) + 1j * g.uniform(-1, 1, n * bs)).astype(dt)
+        cases.append((n, bs, dbl, x, np.empty_like(x)))
+    while elapsed < target_s and reps < 50:
+        for n, bs, dbl, x, y in cases:
+            lens = (ctypes.c_uint64 * 3)(n, 1, 1)
+            fn = lib.ref_portable_exec_f64 if dbl else lib.ref_portable_exec_f32
+            t0 = time.perf_counter()
+            fn(1, lens, bs, -1, 3, x.ctypes.data, y.ctypes.data)
+            fn(1, lens, bs, 1, 3, y.ctypes.data, x.ctypes.data)
+            elapsed += time.perf_counter() - t0
+            total_flops += 2 * flops(n, bs)
+        reps += 1
+    sample = ", ".join(f"{'f64' if d else 'f32'} N={n} batch={bs}" for n, bs, d, _, _ in cases)
+    return total_flops / elapsed / 1e9, reps, sample
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every transform of this workload fits one MI355X and "
+              "multi-GPU runs are not part of this build (--gpus 1 only).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import syclfft_hip as sh
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    lib = oracle()
+    stream = torch.cuda.current_stream(dev)
+
+    # ---------------- setup: plans and device-resident inputs
+    g = torch.Generator(device=dev)
+    g.manual_seed(20260001)
+    subs = []
+    for name, n, b, dbl in WORKLOAD:
+        rdt = torch.float64 if dbl else torch.float32
+        x = (torch.rand((n * b, 2), generator=g, device=dev, dtype=rdt) * 2 - 1)
+        x = torch.view_as_complex(x.contiguous())
+        fwd = sh.Plan([n], b, sh.FORWARD, double=dbl, normalization="orthogonal")
+        bwd = sh.Plan([n], b, sh.BACKWARD, double=dbl, normalization="orthogonal")
+        subs.append(dict(name=name, n=n, b=b, dbl=dbl, x=x, y=torch.empty_like(x),
+                         z=torch.empty_like(x), fwd=fwd, bwd=bwd))
+        log(f"{name}: fwd plan {fwd.describe().strip()}")
+
+    def step():
+        for s in subs:
+            s["fwd"].execute(s["x"], s["y"])
+            s["bwd"].execute(s["y"], s["z"])
+
+    # dominant kernel: the single-launch plan with the most bytes per launch
+    dom = max((s for s in subs if s["fwd"].launch_count() == 1),
+              key=lambda s: s["n"] * s["b"] * (16 if s["dbl"] else 8))
+    dom_bytes = 2 * dom["n"] * dom["b"] * (16 if dom["dbl"] else 8)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    ev = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        for s in subs:
+            if s is dom:
+                a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                s["fwd"].execute(s["x"], s["y"])
+                b_.record(stream)
+                ev.append((a, b_))
+            else:
+                s["fwd"].execute(s["x"], s["y"])
+            s["bwd"].execute(s["y"], s["z"])
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    ms_per_step = gpu_ms / args.steps
+    step_flops = sum(2 * flops(s["n"], s["b"]) for s in subs)
+    step_bytes = sum(2 * 2 * s["n"] * s["b"] * (16 if s["dbl"] else 8) for s in subs)
+    value = step_flops / (ms_per_step * 1e-3) / 1e9
+    dom_ms = float(np.mean([a.elapsed_time(b_) for a, b_ in ev]))
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    # ---------------- accuracy of each sub-workload on sampled lines vs the FP64 oracle
+    acc = {}
+    for s in subs:
+        n, lines = s["n"], 4
+        x = s["x"][: n * lines].cpu().numpy()
+        y = s["y"][: n * lines].cpu().numpy()
+        z = s["z"][: n * lines].cpu().numpy()
+        ref = oracle_fft(lib, x, n, lines, -1, 3)
+        acc[s["name"]] = {
+            "fwd_rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+            "roundtrip_rel_l2": float(np.linalg.norm(z - x) / np.linalg.norm(x)),
+        }
+
+    # ---------------- fp32 batched HBM sweep (512 MiB per buffer)
+    sweep = {}
+    if not args.no_sweep:
+        buf = 512 << 20
+        xs = torch.empty(buf // 8, dtype=torch.complex64, device=dev)
+        xs.real.uniform_(-1, 1)
+        xs.imag.uniform_(-1, 1)
+        ys = torch.empty_like(xs)
+        for lg in range(12, 21):
+            n = 1 << lg
+            b = (buf // 8) // n
+            p = sh.Plan([n], b, sh.FORWARD)
+            for _ in range(2):
+                p.execute(xs, ys)
+            a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 10
+            a.record(stream)
+            for _ in range(reps):
+                p.execute(xs, ys)
+            b_.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b_) / reps
+            gbs = 2 * buf / (ms * 1e-3) / 1e9
+            sweep[str(n)] = {"batch": b, "ms": round(ms, 4), "gflops": round(flops(n, b) / (ms * 1e-3) / 1e9, 1),
+                             "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "launches": p.launch_count()}
+            p.close()
+        del xs, ys
+
+    # ---------------- CPU baseline (reference algorithm, restated; host cores)
+    cpu = None
+    if not args.no_cpu:
+        gf, reps, sample = cpu_baseline(lib)
+        cpu = {"value": round(gf, 3), "unit": "GFLOP/s", "cores": int(lib.oracle_max_threads()),
+               "kind": "port",
+               "sample": f"reference portable-provider algorithm restated in C/OpenMP "
+                         f"(oracle/oracle.c ref_portable_exec_*), fwd+bwd orthogonal, {sample} "
+                         f"(1/16 of the workload's lines), {reps} reps"}
+
+    line = {
+        "metric": "C2C GFLOP/s (5N*log2N), mixed-radix + Bluestein round trip",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic uniform[-1,1) complex, seeded, device-resident",
+        "config": {"workload": "c2c fwd+bwd orthogonal: f64 N=2520 B=1024, f64 N=6561 B=512, "
+                               "f32 N=10007 B=128 (Bluestein)",
+                   "hbm_frac_algorithmic": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "wall_s": round(wall, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"tile_fft {dom['name']} forward (1 launch)",
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4)},
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+        "fp32_hbm_sweep": sweep,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

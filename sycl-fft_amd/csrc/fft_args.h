@@ -1,0 +1,100 @@
+// Kernel-argument layout shared by the plan compiler (host) and the tile kernels (device).
+//
+// One launch of the tile kernel performs, for every "column" of a 2-D tile held in LDS,
+// one FFT of length L (optionally followed by a pointwise product and a second FFT of
+// the same length), with fused pre-ops on load and post-ops on store.  Every transform
+// the reference executes with one or more `parallel_for` passes
+// (src/sycl/sycl.cpp:87-339: copy_scaled, mixed-radix axis, radix-2 axis, Bluestein axis)
+// is expressed as one to three launches of this kernel; the plan compiler in
+// fft_plan.cpp chooses the tiles, strides and fusions.
+#pragma once
+
+#include <stdint.h>
+
+#define SFH_MAX_PASSES 32 /* FFT #1 passes followed by FFT #2 passes */
+
+// Division by a runtime constant 1 <= d < 2^31 for dividends n < 2^31:
+// q = (umulhi(n, m) + n) >> s   with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+struct sfh_fastdiv
+{
+    uint32_t d;
+    uint32_t m;
+    uint32_t s;
+};
+
+// One Stockham radix-r pass.  `ns` is the product of the radices already applied;
+// `twstride` = L / (ns * r) maps the pass twiddle exponent j*t onto the length-L table.
+struct sfh_pass
+{
+    uint32_t radix;
+    uint32_t ns;
+    uint32_t twstride;
+    sfh_fastdiv ns_div;
+};
+
+// Compile-time fusion modes (template parameter of the kernel).
+enum sfh_mode : int
+{
+    SFH_M_PRE = 1,  // multiply loaded element j by pre[p], zero-fill p >= nvalid_in
+    SFH_M_MID = 2,  // after FFT #1: multiply by mid[s], conjugate, run FFT #2
+    SFH_M_T4 = 4,   // after the last FFT: multiply output k by W_N^(col * k)
+    SFH_M_POST = 8  // multiply stored element by post[q], skip q >= nvalid_out
+};
+
+// Run-time flags.
+enum sfh_flags : uint32_t
+{
+    SFH_CONJ_IN = 1u << 0,   // conjugate every loaded element (backward transforms)
+    SFH_CONJ_POST = 1u << 1, // conjugate after the last FFT, before POST (undoes conj-FFT-conj)
+    SFH_CONJ_OUT = 1u << 2,  // conjugate every stored element
+    SFH_SCALE = 1u << 3,     // multiply stored element by `scale`
+    SFH_WFAST = 1u << 4,     // lane mapping: consecutive lanes take consecutive columns
+    SFH_LOAD_LDS = 1u << 5,  // load through a coalescing LDS loop instead of into pass 0
+    SFH_STORE_LDS = 1u << 6  // store through a coalescing LDS loop instead of from the last pass
+};
+
+// Load pipeline:  read -> CONJ_IN -> [PRE: zero if p >= nvalid_in, else * pre[p]]
+// Mid pipeline:   * mid[s] -> conjugate -> FFT #2
+// Store pipeline: [T4] -> CONJ_POST -> [POST: skip if q >= nvalid_out, else * post[q]]
+//                 -> SCALE -> CONJ_OUT -> write
+// with p = j * pre_jmul + col * pre_cmul, s = k * mid_kmul + col * mid_cmul,
+//      q = k * post_kmul + col * post_cmul, T4 exponent = col * k,
+//      col = (wg / col_div) % col_mod + blo * col_bmul, wg = global column index.
+
+struct sfh_tile_args
+{
+    const void *in;
+    void *out;
+    const void *tw;   // W_L^m, m in [0, L): the in-LDS pass twiddles (forward sign)
+    const void *pre;  // pre-op table
+    const void *mid;  // mid-op table (spectrum with 1/M folded in)
+    const void *post; // post-op table
+    const void *t4lo; // W_N^m for m in [0, 2^t4bits)
+    const void *t4hi; // W_N^(m << t4bits)
+
+    // tile index g -> (wt, blo, bhi): wt fastest, then blo, then bhi
+    uint64_t wcount; // extent of the column dimension (ragged last tile is masked)
+    uint64_t nwt;    // number of column tiles = ceil(wcount / W)
+    uint64_t nblo;   // extent of the low batch dimension
+    uint64_t ntiles; // nwt * nblo * nbhi
+    int64_t in_blo, in_bhi, out_blo, out_bhi; // element strides of the batch dimensions
+    int32_t in_w, in_j, out_w, out_k;         // element strides inside a tile (< 2^31 span)
+
+    uint32_t L;      // FFT length held in LDS
+    uint32_t LP;     // padded LDS column pitch (elements)
+    uint32_t W;      // columns per tile (power of two)
+    uint32_t logW;
+    uint32_t tpc;    // threads per column = NT / W
+    uint32_t logtpc;
+    uint32_t npass;  // passes of FFT #1 (FFT #2 uses the reversed radix order)
+    uint32_t flags;
+    sfh_fastdiv L_div;
+    sfh_fastdiv col_div, col_mod;
+    uint32_t col_bmul;
+    uint32_t pre_jmul, pre_cmul, nvalid_in;
+    uint32_t mid_kmul, mid_cmul;
+    uint32_t post_kmul, post_cmul, nvalid_out;
+    uint32_t t4bits;
+    double scale;
+    sfh_pass pass[SFH_MAX_PASSES];
+};

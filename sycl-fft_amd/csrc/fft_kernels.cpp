@@ -1,0 +1,88 @@
+// Variant table and launcher for the tile kernels instantiated in fft_variant.hip.
+#include "fft_kernels.h"
+
+#include <mutex>
+
+#define SFH_DECLARE(name) extern "C" void sfh_kernels_##name(const void **out);
+SFH_DECLARE(f32_64_16)
+SFH_DECLARE(f32_128_16)
+SFH_DECLARE(f32_256_16)
+SFH_DECLARE(f32_512_16)
+SFH_DECLARE(f32_1024_16)
+SFH_DECLARE(f64_64_16)
+SFH_DECLARE(f64_128_16)
+SFH_DECLARE(f64_256_16)
+SFH_DECLARE(f64_512_16)
+SFH_DECLARE(f64_640_16)
+#undef SFH_DECLARE
+
+namespace
+{
+struct variant_spec
+{
+    void (*fill)(const void **);
+    int nt;
+    int e;
+};
+
+const variant_spec k_f32[] = {
+    {sfh_kernels_f32_64_16, 64, 16},    {sfh_kernels_f32_128_16, 128, 16},
+    {sfh_kernels_f32_256_16, 256, 16},  {sfh_kernels_f32_512_16, 512, 16},
+    {sfh_kernels_f32_1024_16, 1024, 16},
+};
+const variant_spec k_f64[] = {
+    {sfh_kernels_f64_64_16, 64, 16},   {sfh_kernels_f64_128_16, 128, 16},
+    {sfh_kernels_f64_256_16, 256, 16}, {sfh_kernels_f64_512_16, 512, 16},
+    {sfh_kernels_f64_640_16, 640, 16},
+};
+
+constexpr int k_nf32 = sizeof(k_f32) / sizeof(k_f32[0]);
+constexpr int k_nf64 = sizeof(k_f64) / sizeof(k_f64[0]);
+sfh_kernel_variant g_f32[k_nf32];
+sfh_kernel_variant g_f64[k_nf64];
+std::once_flag g_once;
+
+void init_tables()
+{
+    for (int i = 0; i < k_nf32; ++i) {
+        k_f32[i].fill(g_f32[i].fn);
+        g_f32[i].nt = k_f32[i].nt;
+        g_f32[i].e = k_f32[i].e;
+    }
+    for (int i = 0; i < k_nf64; ++i) {
+        k_f64[i].fill(g_f64[i].fn);
+        g_f64[i].nt = k_f64[i].nt;
+        g_f64[i].e = k_f64[i].e;
+    }
+}
+} // namespace
+
+extern "C" const sfh_kernel_variant *sfh_variants(int is_double, int *count)
+{
+    std::call_once(g_once, init_tables);
+    if (is_double) {
+        *count = k_nf64;
+        return g_f64;
+    }
+    *count = k_nf32;
+    return g_f32;
+}
+
+extern "C" int sfh_launch_tile(
+    const sfh_kernel_variant *variant, int mode_index, const sfh_tile_args *args,
+    hipStream_t stream)
+{
+    if (args->ntiles == 0) {
+        return 0;
+    }
+    // AQL dispatches carry the grid size in work-items as a 32-bit value
+    if (args->ntiles * static_cast<uint64_t>(variant->nt) > 0xffffffffull || mode_index < 0
+        || mode_index >= SFH_NUM_MODES) {
+        return static_cast<int>(hipErrorInvalidConfiguration);
+    }
+    void *kernel_args[] = {const_cast<sfh_tile_args *>(args)};
+    const hipError_t st = hipLaunchKernel(
+        variant->fn[mode_index], dim3(static_cast<uint32_t>(args->ntiles)),
+        dim3(static_cast<uint32_t>(variant->nt)), kernel_args, 0, stream);
+    return static_cast<int>(st);
+}

@@ -1,0 +1,1237 @@
+// Plan compiler and executor of the MI355X provider (C ABI: include/syclfft_hip_provider.h).
+//
+// Reference behaviour reproduced here:
+//   * validation: checked_element_count (src/core/provider_loader.cpp:112-132) and the
+//     direction check (src/sycl/sycl.cpp:435-438);
+//   * axis order 0..rank-1, stride = product of trailing lengths (src/sycl/sycl.cpp:63-70,
+//     692-767);
+//   * algorithm per axis: power of two / 7-smooth / otherwise Bluestein, decided by trial
+//     division by 2, 3, 5, 7 (supported_factors, src/sycl/sycl.cpp:48-61, 703, 715);
+//   * normalization scale (normalization_scale, src/core/provider_loader.cpp:134-147);
+//   * serialized executions of one plan (src/sycl/sycl.cpp:637-641, 774).
+// Differences by design: every axis runs in 1 launch (fits LDS), 2 launches (four-step
+// split) or 3 launches (Bluestein with a convolution longer than LDS); the scale and the
+// direction are fused into the first/last launch; Bluestein uses the smallest 7-smooth
+// convolution length >= 2n-1 instead of the next power of two.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fft_args.h"
+#include "fft_kernels.h"
+#include "fft_tables.h"
+#include "syclfft_hip_provider.h"
+
+namespace sfh
+{
+
+// ------------------------------------------------------------------------------------
+// Errors
+// ------------------------------------------------------------------------------------
+
+struct sfh_error : std::runtime_error
+{
+    int status;
+    sfh_error(int st, const std::string &msg)
+      : std::runtime_error(msg)
+      , status(st)
+    {}
+};
+
+static void set_err(char *err, size_t cap, const std::string &msg)
+{
+    if (!err || cap == 0) {
+        return;
+    }
+    const size_t n = std::min(cap - 1, msg.size());
+    std::memcpy(err, msg.data(), n);
+    err[n] = '\0';
+}
+
+static void hip_check(hipError_t st, const char *what)
+{
+    if (st != hipSuccess) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_EXECUTION_FAILED,
+            std::string(what) + ": " + hipGetErrorString(st));
+    }
+}
+
+static void trace(const std::string &msg)
+{
+    const char *enabled = std::getenv("SYCLFFT_TRACE");
+    if (!enabled || enabled[0] == '\0' || enabled[0] == '0') {
+        return;
+    }
+    std::fprintf(stderr, "[syclfft] hip provider: %s\n", msg.c_str());
+    std::fflush(stderr);
+}
+
+// ------------------------------------------------------------------------------------
+// Integer helpers
+// ------------------------------------------------------------------------------------
+
+sfh_fastdiv make_fastdiv(uint32_t d)
+{
+    if (d == 0 || d >= (1u << 31)) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "divisor out of range");
+    }
+    uint32_t s = 0;
+    while ((uint64_t(1) << s) < d) {
+        ++s;
+    }
+    const uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1;
+    return {d, static_cast<uint32_t>(m), s};
+}
+
+static bool is_pow2(uint64_t v)
+{
+    return v && !(v & (v - 1));
+}
+
+static uint32_t ilog2(uint64_t v)
+{
+    uint32_t r = 0;
+    while ((uint64_t(1) << r) < v) {
+        ++r;
+    }
+    return r;
+}
+
+// supported_factors (src/sycl/sycl.cpp:48-61): 7-smooth test by trial division.
+static bool is_smooth7(uint64_t n)
+{
+    for (uint64_t r : {2u, 3u, 5u, 7u}) {
+        while (n % r == 0) {
+            n /= r;
+        }
+    }
+    return n == 1;
+}
+
+// Radix program for a 7-smooth length: radix-16 passes for the power-of-two part (then
+// 8/4/2 for the remainder), then 3s, 5s and 7s.
+static std::vector<uint32_t> radix_plan(uint64_t n)
+{
+    std::vector<uint32_t> r;
+    uint32_t a = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        ++a;
+    }
+    while (a >= 4) {
+        r.push_back(16);
+        a -= 4;
+    }
+    if (a == 3) {
+        r.push_back(8);
+    } else if (a == 2) {
+        r.push_back(4);
+    } else if (a == 1) {
+        r.push_back(2);
+    }
+    for (uint32_t p : {3u, 5u, 7u}) {
+        while (n % p == 0) {
+            r.push_back(p);
+            n /= p;
+        }
+    }
+    if (n != 1) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "length is not 7-smooth");
+    }
+    return r;
+}
+
+// Smallest 2^a 3^b 5^c 7^d >= x satisfying pred.
+template <class Pred>
+static uint64_t smooth_at_least(uint64_t x, Pred pred)
+{
+    uint64_t best = 0;
+    for (uint64_t p7 = 1; p7 < 4 * x; p7 *= 7) {
+        for (uint64_t p5 = p7; p5 < 4 * x; p5 *= 5) {
+            for (uint64_t p3 = p5; p3 < 4 * x; p3 *= 3) {
+                uint64_t v = p3;
+                while (v < x) {
+                    v *= 2;
+                }
+                if ((best == 0 || v < best) && pred(v)) {
+                    best = v;
+                }
+            }
+        }
+    }
+    return best;
+}
+
+// ------------------------------------------------------------------------------------
+// Plan
+// ------------------------------------------------------------------------------------
+
+enum buf_id
+{
+    BUF_IN = 0,
+    BUF_OUT = 1,
+    BUF_SCRATCH = 2
+};
+
+struct launch
+{
+    int variant{};
+    int mode{};
+    sfh_tile_args args{};
+    buf_id src{BUF_IN}, dst{BUF_OUT};
+    // table offsets into the plan's table block (patched into args at plan time)
+    int64_t tw_off{-1}, pre_off{-1}, mid_off{-1}, post_off{-1}, t4lo_off{-1}, t4hi_off{-1};
+    std::string desc;
+};
+
+struct plan
+{
+    syclfft_hip_plan_config_v1 cfg{};
+    int device{};
+    bool dbl{};
+    size_t esize{};
+    uint64_t total{};     // batch * prod(lengths)
+    uint64_t transform{}; // prod(lengths)
+    double scale{1.0};
+    std::vector<launch> launches;
+    std::vector<unsigned char> host_tables;
+    std::map<std::string, int64_t> table_cache;
+    void *tables{};
+    uint64_t scratch_elems{};
+    void *scratch{};
+    hipEvent_t last{};
+    bool have_last{};
+    std::mutex mu;
+    const sfh_kernel_variant *variants{};
+    int nvariants{};
+    int copy_only{}; // all lengths 1: a scaled copy
+    int cap_limit{};  // largest tile (complex elements) the planner may use; 0 = no limit
+};
+
+static bool variant_allowed(const plan &p, int i)
+{
+    return p.cap_limit <= 0 || p.variants[i].nt * p.variants[i].e <= p.cap_limit;
+}
+
+static int max_cap(const plan &p)
+{
+    int best = 0;
+    for (int i = 0; i < p.nvariants; ++i) {
+        if (variant_allowed(p, i)) {
+            best = std::max(best, p.variants[i].nt * p.variants[i].e);
+        }
+    }
+    return best;
+}
+
+static int slack_of(const plan &p, int cap)
+{
+    return (static_cast<size_t>(cap + 64) * p.esize > 163840) ? 0 : 64;
+}
+
+// Smallest variant holding W columns of length L (pitch L+1 when W > 1).
+static int pick_variant(const plan &p, uint32_t L, uint32_t W)
+{
+    int best = -1;
+    for (int i = 0; i < p.nvariants; ++i) {
+        if (!variant_allowed(p, i)) {
+            continue;
+        }
+        const int cap = p.variants[i].nt * p.variants[i].e;
+        const uint64_t pitch = W > 1 ? L + 1 : L;
+        if (static_cast<uint64_t>(L) * W > static_cast<uint64_t>(cap)) {
+            continue;
+        }
+        if (pitch * W > static_cast<uint64_t>(cap + slack_of(p, cap))) {
+            continue;
+        }
+        if (static_cast<uint32_t>(p.variants[i].nt) < W) {
+            continue;
+        }
+        if (W > 1 && !is_pow2(static_cast<uint64_t>(p.variants[i].nt))) {
+            continue;
+        }
+        if (best < 0 || cap < p.variants[best].nt * p.variants[best].e) {
+            best = i;
+        }
+    }
+    return best;
+}
+
+template <class T>
+static int64_t add_table(plan &p, const std::string &key, const std::vector<cxl> &values)
+{
+    const auto it = p.table_cache.find(key);
+    if (it != p.table_cache.end()) {
+        return it->second;
+    }
+    size_t off = (p.host_tables.size() + 255) & ~size_t(255);
+    p.host_tables.resize(off + values.size() * 2 * sizeof(T));
+    T *dst = reinterpret_cast<T *>(p.host_tables.data() + off);
+    for (size_t i = 0; i < values.size(); ++i) {
+        dst[2 * i] = static_cast<T>(values[i].re);
+        dst[2 * i + 1] = static_cast<T>(values[i].im);
+    }
+    p.table_cache[key] = static_cast<int64_t>(off);
+    return static_cast<int64_t>(off);
+}
+
+static int64_t table(plan &p, const std::string &key, const std::vector<cxl> &values)
+{
+    return p.dbl ? add_table<double>(p, key, values) : add_table<float>(p, key, values);
+}
+
+static int64_t twiddle_table(plan &p, uint64_t L)
+{
+    const std::string key = "tw" + std::to_string(L);
+    if (p.table_cache.count(key)) {
+        return p.table_cache[key];
+    }
+    std::vector<cxl> v(L);
+    for (uint64_t m = 0; m < L; ++m) {
+        v[m] = root_ld(m, L);
+    }
+    return table(p, key, v);
+}
+
+static void t4_tables(plan &p, uint64_t N, launch &l)
+{
+    const uint32_t bits = (ilog2(N) + 1) / 2;
+    const uint64_t lo_n = uint64_t(1) << bits;
+    const uint64_t hi_n = (N + lo_n - 1) / lo_n;
+    const std::string klo = "t4lo" + std::to_string(N), khi = "t4hi" + std::to_string(N);
+    if (!p.table_cache.count(klo)) {
+        std::vector<cxl> lo(lo_n), hi(hi_n);
+        for (uint64_t m = 0; m < lo_n; ++m) {
+            lo[m] = root_ld(m, N);
+        }
+        for (uint64_t m = 0; m < hi_n; ++m) {
+            hi[m] = root_ld(m * lo_n, N);
+        }
+        table(p, klo, lo);
+        table(p, khi, hi);
+    }
+    l.t4lo_off = p.table_cache[klo];
+    l.t4hi_off = p.table_cache[khi];
+    l.args.t4bits = bits;
+}
+
+static void set_passes(launch &l, uint32_t L, bool second_fft)
+{
+    const std::vector<uint32_t> radices = radix_plan(L);
+    const uint32_t np = static_cast<uint32_t>(radices.size());
+    if (np == 0 || (second_fft ? 2 * np : np) > SFH_MAX_PASSES) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "unsupported radix program");
+    }
+    auto fill = [&](uint32_t base, const std::vector<uint32_t> &rs) {
+        uint32_t ns = 1;
+        for (uint32_t i = 0; i < rs.size(); ++i) {
+            sfh_pass &ps = l.args.pass[base + i];
+            ps.radix = rs[i];
+            ps.ns = ns;
+            ps.twstride = L / (ns * rs[i]);
+            ps.ns_div = make_fastdiv(ns);
+            ns *= rs[i];
+        }
+    };
+    fill(0, radices);
+    if (second_fft) {
+        std::vector<uint32_t> rev(radices.rbegin(), radices.rend());
+        fill(np, rev);
+    }
+    l.args.npass = np;
+    l.args.L = L;
+    l.args.L_div = make_fastdiv(L);
+}
+
+struct geometry
+{
+    uint32_t L{}, W{};
+    uint64_t wcount{}, nblo{1}, nbhi{1};
+    int64_t in_w{}, in_j{}, in_blo{}, in_bhi{};
+    int64_t out_w{}, out_k{}, out_blo{}, out_bhi{};
+    uint64_t col_div{1}, col_mod{1}, col_bmul{0};
+};
+
+static void check_span(const geometry &g)
+{
+    const auto span = [&](int64_t sw, int64_t sj) {
+        return std::llabs(sw) * (g.W - 1) + std::llabs(sj) * (g.L - 1);
+    };
+    if (span(g.in_w, g.in_j) >= (int64_t(1) << 31) || span(g.out_w, g.out_k) >= (int64_t(1) << 31)
+        || g.wcount >= (uint64_t(1) << 31) || g.col_div >= (uint64_t(1) << 31)
+        || g.col_mod >= (uint64_t(1) << 31)) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_PLANNING_FAILED, "transform geometry exceeds 32-bit tile offsets");
+    }
+}
+
+static launch make_launch(plan &p, int mode, const geometry &g, buf_id src, buf_id dst,
+                          bool second_fft, const std::string &what)
+{
+    check_span(g);
+    launch l;
+    l.mode = mode;
+    l.src = src;
+    l.dst = dst;
+    sfh_tile_args &a = l.args;
+    set_passes(l, g.L, second_fft);
+    l.variant = pick_variant(p, g.L, g.W);
+    if (l.variant < 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "no kernel variant fits the tile");
+    }
+    const sfh_kernel_variant &v = p.variants[l.variant];
+    a.W = g.W;
+    a.logW = ilog2(g.W);
+    a.LP = g.W > 1 ? g.L + 1 : g.L;
+    a.tpc = static_cast<uint32_t>(v.nt) / g.W;
+    a.logtpc = ilog2(a.tpc);
+    a.wcount = g.wcount;
+    a.nwt = (g.wcount + g.W - 1) / g.W;
+    a.nblo = g.nblo;
+    a.ntiles = a.nwt * g.nblo * g.nbhi;
+    a.in_w = static_cast<int32_t>(g.in_w);
+    a.in_j = static_cast<int32_t>(g.in_j);
+    a.out_w = static_cast<int32_t>(g.out_w);
+    a.out_k = static_cast<int32_t>(g.out_k);
+    a.in_blo = g.in_blo;
+    a.in_bhi = g.in_bhi;
+    a.out_blo = g.out_blo;
+    a.out_bhi = g.out_bhi;
+    a.col_div = make_fastdiv(static_cast<uint32_t>(g.col_div));
+    a.col_mod = make_fastdiv(static_cast<uint32_t>(g.col_mod));
+    a.col_bmul = static_cast<uint32_t>(g.col_bmul);
+    a.scale = 1.0;
+    // lane mapping: consecutive lanes on consecutive columns when the source is
+    // column-contiguous; otherwise along the column.
+    const bool in_wc = (g.in_w == 1 && g.W > 1);
+    const bool out_wc = (g.out_w == 1 && g.W > 1);
+    if (in_wc) {
+        a.flags |= SFH_WFAST;
+        if (!out_wc) {
+            a.flags |= SFH_STORE_LDS;
+        }
+    } else if (out_wc) {
+        a.flags |= SFH_STORE_LDS;
+    }
+    l.tw_off = twiddle_table(p, g.L);
+    char buf[256];
+    std::snprintf(
+        buf, sizeof(buf), "%s L=%u W=%u tiles=%llu nt=%d e=%d mode=%d flags=0x%x", what.c_str(),
+        g.L, g.W, static_cast<unsigned long long>(a.ntiles), v.nt, v.e, mode, a.flags);
+    l.desc = buf;
+    return l;
+}
+
+// Columns per tile for an axis whose columns are `contig` (adjacent in memory) or whose
+// points are contiguous.
+static uint32_t choose_w(const plan &p, uint64_t L, uint64_t wcount, bool contig_cols)
+{
+    const uint64_t cap = static_cast<uint64_t>(max_cap(p));
+    uint64_t w;
+    if (contig_cols) {
+        w = 16;
+        while (w < 64 && w * 2 * L <= 2048) {
+            w *= 2;
+        }
+    } else {
+        w = 1;
+        while (w < 64 && w * L < 1024) {
+            w *= 2;
+        }
+    }
+    while (w > 1 && w * (L + 1) > cap) {
+        w /= 2;
+    }
+    // do not make tiles wider than the column dimension needs
+    while (w > 1 && w / 2 >= wcount) {
+        w /= 2;
+    }
+    return static_cast<uint32_t>(w);
+}
+
+static bool fits_single(const plan &p, uint64_t L, uint32_t W)
+{
+    return L <= static_cast<uint64_t>(max_cap(p)) && pick_variant(p, static_cast<uint32_t>(L), W) >= 0;
+}
+
+// Split n = n1 * n2 (n1 <= n2) with both sides fitting a tile of `wmin` columns.
+static bool split_two(const plan &p, uint64_t n, uint32_t wmin, uint64_t &n1, uint64_t &n2)
+{
+    uint64_t best = 0;
+    for (uint64_t d = 2; d * d <= n; ++d) {
+        if (n % d == 0) {
+            const uint64_t e = n / d;
+            if (fits_single(p, d, wmin) && fits_single(p, e, wmin)) {
+                best = d; // largest d <= sqrt(n) wins
+            }
+        }
+    }
+    if (!best) {
+        return false;
+    }
+    n1 = best;
+    n2 = n / best;
+    return true;
+}
+
+static uint32_t min_cols(const plan &p)
+{
+    return p.dbl ? 8u : 16u;
+}
+
+// One axis of extent n with `outer` lines before and `inner` elements after it.
+static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_id src, buf_id dst)
+{
+    if (n == 1) {
+        return;
+    }
+    if (is_smooth7(n)) {
+        // ---------------- single launch ----------------
+        {
+            geometry g;
+            g.L = static_cast<uint32_t>(std::min<uint64_t>(n, 0xffffffffu));
+            if (inner == 1) {
+                g.W = choose_w(p, n, outer, false);
+                g.wcount = outer;
+                g.in_w = g.out_w = static_cast<int64_t>(n);
+                g.in_j = g.out_k = 1;
+            } else {
+                g.W = choose_w(p, n, inner, true);
+                g.wcount = inner;
+                g.nblo = outer;
+                g.in_w = g.out_w = 1;
+                g.in_j = g.out_k = static_cast<int64_t>(inner);
+                g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
+            }
+            if (n <= static_cast<uint64_t>(max_cap(p)) && fits_single(p, n, g.W)) {
+                p.launches.push_back(make_launch(p, SFH_MI_PLAIN, g, src, dst, false, "single"));
+                return;
+            }
+        }
+        // ---------------- two launches (four-step) ----------------
+        uint64_t n1 = 0, n2 = 0;
+        uint32_t wmin = min_cols(p);
+        while (!split_two(p, n, wmin, n1, n2)) {
+            if (wmin == 1) {
+                throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "transform length too large");
+            }
+            wmin /= 2;
+        }
+        p.scratch_elems = std::max(p.scratch_elems, p.total);
+        {
+            // pass A: FFT over j1 (n1 points, stride n2*inner) for every (outer, j2, inner),
+            // twiddle W_n^(j2 k1), in place positions into scratch.
+            geometry g;
+            g.L = static_cast<uint32_t>(n1);
+            g.wcount = n2 * inner;
+            g.W = choose_w(p, n1, g.wcount, true);
+            g.nblo = outer;
+            g.in_w = g.out_w = 1;
+            g.in_j = g.out_k = static_cast<int64_t>(n2 * inner);
+            g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
+            g.col_div = inner;
+            g.col_mod = n2;
+            launch l = make_launch(p, SFH_MI_T4, g, src, BUF_SCRATCH, false, "fourstep-A");
+            t4_tables(p, n, l);
+            p.launches.push_back(l);
+        }
+        {
+            // pass B: FFT over j2 (n2 points) of row k1, output k1 + n1 k2.
+            geometry g;
+            g.L = static_cast<uint32_t>(n2);
+            if (inner == 1) {
+                g.wcount = n1;
+                g.W = choose_w(p, n2, n1, true);
+                g.nblo = outer;
+                g.in_w = static_cast<int64_t>(n2);
+                g.in_j = 1;
+                g.out_w = 1;
+                g.out_k = static_cast<int64_t>(n1);
+                g.in_blo = g.out_blo = static_cast<int64_t>(n);
+            } else {
+                g.wcount = inner;
+                g.W = choose_w(p, n2, inner, true);
+                g.nblo = n1;
+                g.nbhi = outer;
+                g.in_w = g.out_w = 1;
+                g.in_j = static_cast<int64_t>(inner);
+                g.out_k = static_cast<int64_t>(n1 * inner);
+                g.in_blo = static_cast<int64_t>(n2 * inner);
+                g.out_blo = static_cast<int64_t>(inner);
+                g.in_bhi = g.out_bhi = static_cast<int64_t>(n * inner);
+            }
+            p.launches.push_back(make_launch(p, SFH_MI_PLAIN, g, BUF_SCRATCH, dst, false, "fourstep-B"));
+        }
+        return;
+    }
+
+    // ---------------- Bluestein ----------------
+    // chirp c[j] = exp(-i pi j^2/n); b[m] = conj(c[m]) for |m| < n wrapped mod M;
+    // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
+    const uint64_t need = 2 * n - 1;
+    const uint64_t cap = static_cast<uint64_t>(max_cap(p));
+    uint64_t M = smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+    const bool fused = M != 0;
+    uint64_t m1 = 0, m2 = 0;
+    if (!fused) {
+        uint32_t wmin = min_cols(p);
+        for (;;) {
+            M = smooth_at_least(need, [&](uint64_t m) {
+                uint64_t a, b;
+                return split_two(p, m, wmin, a, b);
+            });
+            if (M) {
+                break;
+            }
+            if (wmin == 1) {
+                throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "Bluestein length too large");
+            }
+            wmin /= 2;
+        }
+        split_two(p, M, wmin, m1, m2);
+    }
+    // tables
+    std::vector<cxl> chirp(n), bvec(M, cxl{0.0L, 0.0L});
+    for (uint64_t j = 0; j < n; ++j) {
+        chirp[j] = chirp_ld(j, n);
+        const cxl b{chirp[j].re, -chirp[j].im};
+        bvec[j] = b;
+        if (j) {
+            bvec[M - j] = b;
+        }
+    }
+    std::vector<cxl> spec = host_dft_ld(bvec);
+    const long double invM = 1.0L / static_cast<long double>(M);
+    for (auto &s : spec) {
+        s.re *= invM;
+        s.im *= invM;
+    }
+    const std::string nkey = std::to_string(n) + "_" + std::to_string(M);
+    const int64_t chirp_off = table(p, "chirp" + nkey, chirp);
+
+    if (fused) {
+        geometry g;
+        g.L = static_cast<uint32_t>(M);
+        if (inner == 1) {
+            g.W = choose_w(p, M, outer, false);
+            while (g.W > 1 && !fits_single(p, M, g.W)) {
+                g.W /= 2;
+            }
+            g.wcount = outer;
+            g.in_w = g.out_w = static_cast<int64_t>(n);
+            g.in_j = g.out_k = 1;
+        } else {
+            g.W = 1;
+            while (g.W < 16 && fits_single(p, M, g.W * 2) && g.W * 2 <= inner) {
+                g.W *= 2;
+            }
+            g.wcount = inner;
+            g.nblo = outer;
+            g.in_w = g.out_w = 1;
+            g.in_j = g.out_k = static_cast<int64_t>(inner);
+            g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
+        }
+        launch l = make_launch(p, SFH_MI_BLUE_FUSED, g, src, dst, true, "bluestein-fused");
+        sfh_tile_args &a = l.args;
+        a.pre_jmul = 1;
+        a.pre_cmul = 0;
+        a.nvalid_in = static_cast<uint32_t>(n);
+        a.mid_kmul = 1;
+        a.mid_cmul = 0;
+        a.post_kmul = 1;
+        a.post_cmul = 0;
+        a.nvalid_out = static_cast<uint32_t>(n);
+        a.flags |= SFH_CONJ_POST;
+        l.pre_off = chirp_off;
+        l.post_off = chirp_off;
+        l.mid_off = table(p, "spec" + nkey, spec);
+        p.launches.push_back(l);
+        return;
+    }
+
+    // three launches over a (outer, M, inner) scratch, M = m1 * m2
+    p.scratch_elems = std::max(p.scratch_elems, outer * M * inner);
+    {
+        // A: a[j] = x[j] c[j] (j < n), column FFT over j1 of j = j1 m2 + j2, twiddle W_M^(j2 k1)
+        geometry g;
+        g.L = static_cast<uint32_t>(m1);
+        g.wcount = m2 * inner;
+        g.W = choose_w(p, m1, g.wcount, true);
+        g.nblo = outer;
+        g.in_w = 1;
+        g.in_j = static_cast<int64_t>(m2 * inner);
+        g.in_blo = static_cast<int64_t>(n * inner);
+        g.out_w = 1;
+        g.out_k = static_cast<int64_t>(m2 * inner);
+        g.out_blo = static_cast<int64_t>(M * inner);
+        g.col_div = inner;
+        g.col_mod = m2;
+        launch l = make_launch(p, SFH_MI_BLUE_A, g, src, BUF_SCRATCH, false, "bluestein-A");
+        l.args.pre_jmul = static_cast<uint32_t>(m2);
+        l.args.pre_cmul = 1;
+        l.args.nvalid_in = static_cast<uint32_t>(n);
+        l.pre_off = chirp_off;
+        t4_tables(p, M, l);
+        p.launches.push_back(l);
+    }
+    {
+        // B: row k1: FFT over j2, * spectrum (stored transposed), conj, FFT, twiddle W_M^(k1 j2)
+        std::vector<cxl> spec_t(M);
+        for (uint64_t k1 = 0; k1 < m1; ++k1) {
+            for (uint64_t k2 = 0; k2 < m2; ++k2) {
+                spec_t[k1 * m2 + k2] = spec[k1 + m1 * k2];
+            }
+        }
+        geometry g;
+        g.L = static_cast<uint32_t>(m2);
+        if (inner == 1) {
+            g.wcount = m1;
+            g.W = choose_w(p, m2, m1, false);
+            g.nblo = outer;
+            g.in_w = g.out_w = static_cast<int64_t>(m2);
+            g.in_j = g.out_k = 1;
+            g.in_blo = g.out_blo = static_cast<int64_t>(M);
+            g.col_div = 1;
+            g.col_mod = m1;
+        } else {
+            g.wcount = inner;
+            g.W = choose_w(p, m2, inner, true);
+            g.nblo = m1;
+            g.nbhi = outer;
+            g.in_w = g.out_w = 1;
+            g.in_j = g.out_k = static_cast<int64_t>(inner);
+            g.in_blo = g.out_blo = static_cast<int64_t>(m2 * inner);
+            g.in_bhi = g.out_bhi = static_cast<int64_t>(M * inner);
+            g.col_div = 1;
+            g.col_mod = 1;
+            g.col_bmul = 1;
+        }
+        launch l = make_launch(p, SFH_MI_BLUE_B, g, BUF_SCRATCH, BUF_SCRATCH, true, "bluestein-B");
+        l.args.mid_kmul = 1;
+        l.args.mid_cmul = static_cast<uint32_t>(m2);
+        l.mid_off = table(p, "spect" + nkey, spec_t);
+        t4_tables(p, M, l);
+        p.launches.push_back(l);
+    }
+    {
+        // C: column FFT over k1 -> conv index j = j2 + m2 j1; conj, * c[j], keep j < n
+        geometry g;
+        g.L = static_cast<uint32_t>(m1);
+        g.wcount = m2 * inner;
+        g.W = choose_w(p, m1, g.wcount, true);
+        g.nblo = outer;
+        g.in_w = 1;
+        g.in_j = static_cast<int64_t>(m2 * inner);
+        g.in_blo = static_cast<int64_t>(M * inner);
+        g.out_w = 1;
+        g.out_k = static_cast<int64_t>(m2 * inner);
+        g.out_blo = static_cast<int64_t>(n * inner);
+        g.col_div = inner;
+        g.col_mod = m2;
+        launch l = make_launch(p, SFH_MI_BLUE_C, g, BUF_SCRATCH, dst, false, "bluestein-C");
+        l.args.post_kmul = static_cast<uint32_t>(m2);
+        l.args.post_cmul = 1;
+        l.args.nvalid_out = static_cast<uint32_t>(n);
+        l.args.flags |= SFH_CONJ_POST;
+        l.post_off = chirp_off;
+        p.launches.push_back(l);
+    }
+}
+
+// normalization_scale (src/core/provider_loader.cpp:134-147)
+static double norm_scale(uint32_t mode, int32_t dir, uint64_t n)
+{
+    switch (mode) {
+    case SYCLFFT_HIP_NORM_NONE:
+        return 1.0;
+    case SYCLFFT_HIP_NORM_FORWARD:
+        return dir == -1 ? 1.0 / static_cast<double>(n) : 1.0;
+    case SYCLFFT_HIP_NORM_BACKWARD:
+        return dir == 1 ? 1.0 / static_cast<double>(n) : 1.0;
+    case SYCLFFT_HIP_NORM_ORTHOGONAL:
+        return 1.0 / std::sqrt(static_cast<double>(n));
+    default:
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Invalid normalization");
+    }
+}
+
+static void validate_config(const syclfft_hip_plan_config_v1 *cfg)
+{
+    if (!cfg) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plan config must be non-null");
+    }
+    if (cfg->abi_version != SYCLFFT_DEVICE_PROVIDER_ABI_VERSION) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLUGIN_ERROR, "plan config has an incompatible ABI version");
+    }
+    if (cfg->scalar != SYCLFFT_HIP_SCALAR_FLOAT && cfg->scalar != SYCLFFT_HIP_SCALAR_DOUBLE) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_UNSUPPORTED_PRECISION, "scalar must be float or double");
+    }
+    if (cfg->direction != -1 && cfg->direction != 1) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Invalid FFT direction");
+    }
+    // checked_element_count (src/core/provider_loader.cpp:112-132)
+    if (cfg->rank < 1 || cfg->rank > 3) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "FFT rank must be between 1 and 3");
+    }
+    if (cfg->batch_count == 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "FFT batch count must be non-zero");
+    }
+    uint64_t count = cfg->batch_count;
+    for (uint32_t i = 0; i < cfg->rank; ++i) {
+        const uint64_t len = cfg->lengths[i];
+        if (len == 0) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "FFT dimensions must be non-zero");
+        }
+        if (count > UINT64_MAX / len) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "FFT element count overflows size_t");
+        }
+        count *= len;
+    }
+    if (cfg->normalization > SYCLFFT_HIP_NORM_ORTHOGONAL) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Invalid normalization");
+    }
+    if (cfg->precision > SYCLFFT_HIP_PRECISION_SPLIT_FP16) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Invalid precision mode");
+    }
+    if (cfg->precision != SYCLFFT_HIP_PRECISION_VALU && cfg->scalar != SYCLFFT_HIP_SCALAR_FLOAT) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_UNSUPPORTED_PRECISION, "split-precision MFMA modes apply to float plans");
+    }
+    if (cfg->reserved[0] || cfg->reserved[1] || cfg->reserved[2]) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "reserved config fields must be zero");
+    }
+}
+
+static int check_device(int device, std::string &reason)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        reason = "no HIP device is visible";
+        return -1;
+    }
+    int dev = device;
+    if (dev < 0) {
+        if (hipGetDevice(&dev) != hipSuccess) {
+            reason = "hipGetDevice failed";
+            return -1;
+        }
+    }
+    if (dev >= count) {
+        reason = "device ordinal out of range";
+        return -1;
+    }
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        reason = "hipGetDeviceProperties failed";
+        return -1;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        reason = std::string("device architecture ") + prop.gcnArchName
+                 + " is not gfx950 (MI355X); kernels are built for gfx950 only";
+        return -1;
+    }
+    return dev;
+}
+
+struct device_guard
+{
+    int saved{-1};
+    explicit device_guard(int dev)
+    {
+        if (hipGetDevice(&saved) != hipSuccess) {
+            saved = -1;
+        }
+        if (saved != dev) {
+            hip_check(hipSetDevice(dev), "hipSetDevice");
+        }
+    }
+    ~device_guard()
+    {
+        int cur = -1;
+        if (saved >= 0 && hipGetDevice(&cur) == hipSuccess && cur != saved) {
+            (void)hipSetDevice(saved);
+        }
+    }
+};
+
+static void release(plan *p)
+{
+    if (!p) {
+        return;
+    }
+    if (p->have_last) {
+        (void)hipEventSynchronize(p->last);
+    }
+    if (p->last) {
+        (void)hipEventDestroy(p->last);
+    }
+    if (p->scratch) {
+        (void)hipFree(p->scratch);
+    }
+    if (p->tables) {
+        (void)hipFree(p->tables);
+    }
+    delete p;
+}
+
+static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
+{
+    validate_config(cfg);
+    std::string reason;
+    const int dev = check_device(cfg->device, reason);
+    if (dev < 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, reason);
+    }
+    std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
+    p->cfg = *cfg;
+    p->device = dev;
+    p->dbl = cfg->scalar == SYCLFFT_HIP_SCALAR_DOUBLE;
+    p->esize = p->dbl ? 16 : 8;
+    p->variants = sfh_variants(p->dbl ? 1 : 0, &p->nvariants);
+    // SYCLFFT_HIP_MAX_TILE caps the LDS tile (complex elements): a tuning and test knob that
+    // forces the multi-launch paths at small lengths.
+    if (const char *cap = std::getenv("SYCLFFT_HIP_MAX_TILE")) {
+        p->cap_limit = std::atoi(cap);
+    }
+    p->transform = 1;
+    for (uint32_t i = 0; i < cfg->rank; ++i) {
+        p->transform *= cfg->lengths[i];
+    }
+    p->total = p->transform * cfg->batch_count;
+    p->scale = norm_scale(cfg->normalization, cfg->direction, p->transform);
+
+    // axis loop in the reference order (src/sycl/sycl.cpp:692-767)
+    for (uint32_t axis = 0; axis < cfg->rank; ++axis) {
+        uint64_t inner = 1;
+        for (uint32_t k = axis + 1; k < cfg->rank; ++k) {
+            inner *= cfg->lengths[k];
+        }
+        const uint64_t n = cfg->lengths[axis];
+        const uint64_t outer = p->total / (n * inner);
+        const buf_id src = p->launches.empty() ? BUF_IN : BUF_OUT;
+        plan_axis(*p, outer, n, inner, src, BUF_OUT);
+    }
+    if (p->launches.empty()) {
+        p->copy_only = 1; // every length is 1: out = scale * in
+    } else {
+        if (cfg->direction == 1) {
+            p->launches.front().args.flags |= SFH_CONJ_IN;
+            p->launches.back().args.flags |= SFH_CONJ_OUT;
+        }
+        if (p->scale != 1.0) {
+            p->launches.back().args.flags |= SFH_SCALE;
+            p->launches.back().args.scale = p->scale;
+        }
+    }
+
+    device_guard guard(dev);
+    if (!p->host_tables.empty()) {
+        hip_check(hipMalloc(&p->tables, p->host_tables.size()), "hipMalloc(tables)");
+        hip_check(
+            hipMemcpy(p->tables, p->host_tables.data(), p->host_tables.size(), hipMemcpyHostToDevice),
+            "hipMemcpy(tables)");
+    }
+    if (p->scratch_elems) {
+        hip_check(hipMalloc(&p->scratch, p->scratch_elems * p->esize), "hipMalloc(scratch)");
+    }
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
+    auto ptr = [&](int64_t off) -> const void * {
+        return off < 0 ? nullptr : static_cast<const unsigned char *>(p->tables) + off;
+    };
+    for (auto &l : p->launches) {
+        l.args.tw = ptr(l.tw_off);
+        l.args.pre = ptr(l.pre_off);
+        l.args.mid = ptr(l.mid_off);
+        l.args.post = ptr(l.post_off);
+        l.args.t4lo = ptr(l.t4lo_off);
+        l.args.t4hi = ptr(l.t4hi_off);
+    }
+    p->host_tables.clear();
+    p->host_tables.shrink_to_fit();
+    for (const auto &l : p->launches) {
+        trace(l.desc);
+    }
+    return p.release();
+}
+
+// Device-pointer check: the whole [ptr, ptr+bytes) range must lie in one device
+// allocation visible to the plan's device (replaces the USM check of validate_pointer,
+// src/sycl/sycl.cpp:72-85).  A host pointer would fault the GPU, so it is rejected here.
+static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, const char *name)
+{
+    if (!ptr) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " must be non-null");
+    }
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " is not a HIP device allocation");
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " is not device memory");
+    }
+    if (attr.type == hipMemoryTypeDevice && attr.device != p.device) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_POINTER,
+            std::string(name) + " belongs to another device than the plan's");
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(ptr)) == hipSuccess && base) {
+        const auto b = reinterpret_cast<uintptr_t>(base);
+        const auto q = reinterpret_cast<uintptr_t>(ptr);
+        if (q < b || q - b + bytes > size) {
+            throw sfh_error(
+                SYCLFFT_HIP_STATUS_INVALID_POINTER,
+                std::string(name) + " allocation is smaller than the transform");
+        }
+    } else {
+        (void)hipGetLastError();
+    }
+}
+
+static void execute_plan(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
+    hipEvent_t done)
+{
+    const uint64_t bytes = p.total * p.esize;
+    validate_pointer(p, in, bytes, "input");
+    validate_pointer(p, out, bytes, "output");
+    if (p.cfg.in_place && in != out) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "In-place plan must be executed with execute(inout)");
+    }
+    if (!p.cfg.in_place && in == out) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place plan requires distinct allocations");
+    }
+    if (!p.cfg.in_place) {
+        const auto a = reinterpret_cast<uintptr_t>(in), b = reinterpret_cast<uintptr_t>(out);
+        if (a < b + bytes && b < a + bytes) {
+            throw sfh_error(
+                SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
+        }
+    }
+    std::lock_guard<std::mutex> lock(p.mu);
+    device_guard guard(p.device);
+    for (size_t i = 0; i < ndeps; ++i) {
+        if (deps[i]) {
+            hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
+        }
+    }
+    if (p.have_last) {
+        hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
+    }
+    if (p.copy_only) {
+        hip_check(
+            static_cast<hipError_t>(
+                sfh_launch_copy_scale(in, out, p.total, p.dbl ? 1 : 0, p.scale, stream)),
+            "copy_scale launch");
+    }
+    void *bufs[3] = {const_cast<void *>(in), out, p.scratch};
+    for (auto &l : p.launches) {
+        sfh_tile_args a = l.args;
+        a.in = bufs[l.src];
+        a.out = bufs[l.dst];
+        hip_check(
+            static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, stream)),
+            "tile kernel launch");
+    }
+    hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+    p.have_last = true;
+    if (done) {
+        hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+    }
+}
+
+} // namespace sfh
+
+// ------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------
+
+using sfh::sfh_error;
+
+extern "C" int syclfft_hip_supports(int device, char *err, size_t cap)
+{
+    try {
+        std::string reason;
+        if (sfh::check_device(device, reason) < 0) {
+            sfh::set_err(err, cap, reason);
+            return 0;
+        }
+        sfh::set_err(err, cap, "");
+        return 1;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return 0;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return 0;
+    }
+}
+
+extern "C" void *syclfft_hip_plan_create(const syclfft_hip_plan_config_v1 *config, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        return sfh::create_plan(config);
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+    } catch (const std::bad_alloc &) {
+        sfh::set_err(err, cap, "Unable to allocate plan memory");
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+    }
+    return nullptr;
+}
+
+extern "C" int syclfft_hip_plan_create_ex(
+    const syclfft_hip_plan_config_v1 *config, void **plan, char *err, size_t cap)
+{
+    if (plan) {
+        *plan = nullptr;
+    }
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plan output pointer is null");
+        }
+        *plan = sfh::create_plan(config);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::bad_alloc &) {
+        sfh::set_err(err, cap, "Unable to allocate plan memory");
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+    }
+    return SYCLFFT_HIP_STATUS_PLANNING_FAILED;
+}
+
+extern "C" void syclfft_hip_plan_destroy(void *plan)
+{
+    sfh::release(static_cast<sfh::plan *>(plan));
+}
+
+extern "C" uint64_t syclfft_hip_plan_scratch_bytes(const void *plan)
+{
+    if (!plan) {
+        return 0;
+    }
+    const auto *p = static_cast<const sfh::plan *>(plan);
+    return p->scratch_elems * p->esize;
+}
+
+extern "C" int syclfft_hip_plan_execute(
+    void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
+    void *done, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE, "plan handle is null");
+        }
+        if (ndeps && !deps) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "dependency list is null");
+        }
+        sfh::execute_plan(
+            *static_cast<sfh::plan *>(plan), in, out, static_cast<hipStream_t>(stream), deps, ndeps,
+            static_cast<hipEvent_t>(done));
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+extern "C" size_t syclfft_hip_plan_describe(const void *plan, char *buf, size_t cap)
+{
+    if (!plan) {
+        return 0;
+    }
+    const auto *p = static_cast<const sfh::plan *>(plan);
+    std::string s;
+    if (p->copy_only) {
+        s += "copy_scale\n";
+    }
+    for (const auto &l : p->launches) {
+        s += l.desc;
+        s += '\n';
+    }
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return s.size();
+}
+
+extern "C" uint32_t syclfft_hip_plan_launch_count(const void *plan)
+{
+    if (!plan) {
+        return 0;
+    }
+    const auto *p = static_cast<const sfh::plan *>(plan);
+    return static_cast<uint32_t>(p->launches.size()) + (p->copy_only ? 1u : 0u);
+}
+
+namespace
+{
+int api_supports(int device, char *err, size_t cap)
+{
+    return syclfft_hip_supports(device, err, cap);
+}
+void *api_create(const syclfft_hip_plan_config_v1 *config, char *err, size_t cap)
+{
+    return syclfft_hip_plan_create(config, err, cap);
+}
+void api_destroy(void *plan)
+{
+    syclfft_hip_plan_destroy(plan);
+}
+uint64_t api_scratch(const void *plan)
+{
+    return syclfft_hip_plan_scratch_bytes(plan);
+}
+int api_execute(
+    void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
+    void *done, char *err, size_t cap)
+{
+    return syclfft_hip_plan_execute(plan, in, out, stream, deps, ndeps, done, err, cap);
+}
+const syclfft_device_provider_v1 k_api = {
+    SYCLFFT_DEVICE_PROVIDER_ABI_VERSION, "hip_mi355x", api_supports, api_create, api_destroy,
+    api_scratch, api_execute};
+} // namespace
+
+extern "C" const syclfft_device_provider_v1 *syclfft_get_device_provider_v1(void)
+{
+    return &k_api;
+}

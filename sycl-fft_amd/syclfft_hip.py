@@ -1,0 +1,268 @@
+"""ctypes binding of the MI355X syclfft provider's C ABI (include/syclfft_hip_provider.h).
+
+Mirrors the reference plan API's meaning (include/syclfft/syclfft.hpp:45-129,
+include/syclfft/common.hpp:34-110): a plan is built once for (lengths, batch, direction,
+placement, normalization) and executed on device buffers; errors raise `SyclfftError`
+carrying the reference's `error_code` name.  Device memory and streams come from PyTorch
+(tensors' data_ptr() and torch.cuda streams); no computation happens in Python.
+
+The provider library is found in ./lib/syclfft/providers next to this file (built by
+`make` in this directory), or at $SYCLFFT_HIP_PROVIDER.  Loading fails loudly if it is
+missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PROVIDER_PATH = os.path.join(_HERE, "lib", "syclfft", "providers", "syclfft_provider_hip.so")
+
+ABI_VERSION = 1
+SCALAR_FLOAT = 1
+SCALAR_DOUBLE = 2
+FORWARD = -1
+BACKWARD = 1
+NORM = {"none": 0, "forward": 1, "backward": 2, "orthogonal": 3}
+PRECISION = {"valu": 0, "split_bf16": 1, "split_fp16": 2}
+
+# status = 1 + syclfft::error_code (include/syclfft/common.hpp:83-93)
+ERROR_CODES = {
+    1: "invalid_argument",
+    2: "invalid_state",
+    3: "invalid_pointer",
+    4: "unsupported_precision",
+    5: "provider_unavailable",
+    6: "planning_failed",
+    7: "execution_failed",
+    8: "plugin_error",
+}
+
+# every symbol include/syclfft_hip_provider.h declares
+EXPORTED_SYMBOLS = (
+    "syclfft_get_device_provider_v1",
+    "syclfft_hip_supports",
+    "syclfft_hip_plan_create",
+    "syclfft_hip_plan_create_ex",
+    "syclfft_hip_plan_destroy",
+    "syclfft_hip_plan_scratch_bytes",
+    "syclfft_hip_plan_execute",
+    "syclfft_hip_plan_describe",
+    "syclfft_hip_plan_launch_count",
+)
+
+
+class SyclfftError(RuntimeError):
+    """Error raised by the provider; `code` is the syclfft::error_code name."""
+
+    def __init__(self, code: str, message: str):
+        super().__init__(f"{code}: {message}")
+        self.code = code
+        self.message = message
+
+
+class PlanConfig(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_uint32),
+        ("scalar", ctypes.c_uint32),
+        ("direction", ctypes.c_int32),
+        ("rank", ctypes.c_uint32),
+        ("lengths", ctypes.c_uint64 * 3),
+        ("batch_count", ctypes.c_uint64),
+        ("in_place", ctypes.c_uint32),
+        ("normalization", ctypes.c_uint32),
+        ("measure", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+
+_SUPPORTS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t)
+_CREATE = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.POINTER(PlanConfig), ctypes.c_char_p, ctypes.c_size_t)
+_DESTROY = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+_SCRATCH = ctypes.CFUNCTYPE(ctypes.c_uint64, ctypes.c_void_p)
+_EXECUTE = ctypes.CFUNCTYPE(
+    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
+    ctypes.c_size_t)
+
+
+class ProviderTable(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_uint32),
+        ("name", ctypes.c_char_p),
+        ("supports", _SUPPORTS),
+        ("create", _CREATE),
+        ("destroy", _DESTROY),
+        ("scratch_bytes", _SCRATCH),
+        ("execute", _EXECUTE),
+    ]
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load the provider library (raises OSError if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or os.environ.get("SYCLFFT_HIP_PROVIDER", PROVIDER_PATH)
+    if not os.path.exists(p):
+        raise OSError(f"syclfft HIP provider not built: {p} is missing (run make in {_HERE})")
+    lib = ctypes.CDLL(p, mode=ctypes.RTLD_LOCAL)
+    lib.syclfft_hip_supports.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_supports.restype = ctypes.c_int
+    lib.syclfft_hip_plan_create.argtypes = [ctypes.POINTER(PlanConfig), ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_plan_create.restype = ctypes.c_void_p
+    lib.syclfft_hip_plan_create_ex.argtypes = [
+        ctypes.POINTER(PlanConfig), ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_plan_create_ex.restype = ctypes.c_int
+    lib.syclfft_hip_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.syclfft_hip_plan_destroy.restype = None
+    lib.syclfft_hip_plan_scratch_bytes.argtypes = [ctypes.c_void_p]
+    lib.syclfft_hip_plan_scratch_bytes.restype = ctypes.c_uint64
+    lib.syclfft_hip_plan_execute.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
+        ctypes.c_size_t]
+    lib.syclfft_hip_plan_execute.restype = ctypes.c_int
+    lib.syclfft_hip_plan_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_plan_describe.restype = ctypes.c_size_t
+    lib.syclfft_hip_plan_launch_count.argtypes = [ctypes.c_void_p]
+    lib.syclfft_hip_plan_launch_count.restype = ctypes.c_uint32
+    lib.syclfft_get_device_provider_v1.argtypes = []
+    lib.syclfft_get_device_provider_v1.restype = ctypes.POINTER(ProviderTable)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def provider_table(lib: Optional[ctypes.CDLL] = None) -> ProviderTable:
+    lib = lib or load()
+    return lib.syclfft_get_device_provider_v1().contents
+
+
+def supports(device: int = -1) -> tuple[bool, str]:
+    err = ctypes.create_string_buffer(1024)
+    ok = load().syclfft_hip_supports(device, err, 1024)
+    return bool(ok), err.value.decode()
+
+
+def make_config(lengths: Sequence[int], batch: int = 1, direction: int = FORWARD,
+                double: bool = False, in_place: bool = False, normalization: str = "none",
+                precision: str = "valu", measure: bool = False, device: int = -1) -> PlanConfig:
+    cfg = PlanConfig()
+    cfg.abi_version = ABI_VERSION
+    cfg.scalar = SCALAR_DOUBLE if double else SCALAR_FLOAT
+    cfg.direction = int(direction)
+    cfg.rank = len(lengths)
+    for i, n in enumerate(list(lengths)[:3]):
+        cfg.lengths[i] = int(n)
+    cfg.batch_count = int(batch)
+    cfg.in_place = 1 if in_place else 0
+    cfg.normalization = NORM[normalization] if isinstance(normalization, str) else int(normalization)
+    cfg.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    cfg.measure = 1 if measure else 0
+    cfg.device = int(device)
+    return cfg
+
+
+def _raise(status: int, err: ctypes.Array) -> None:
+    raise SyclfftError(ERROR_CODES.get(status, f"status{status}"), err.value.decode())
+
+
+class Plan:
+    """A provider plan (RAII over syclfft_hip_plan_create / _destroy)."""
+
+    def __init__(self, lengths: Sequence[int], batch: int = 1, direction: int = FORWARD,
+                 double: bool = False, in_place: bool = False, normalization: str = "none",
+                 precision: str = "valu", measure: bool = False, device: int = -1):
+        self._lib = load()
+        self.lengths = tuple(int(n) for n in lengths)
+        self.batch = int(batch)
+        self.direction = int(direction)
+        self.double = bool(double)
+        self.in_place = bool(in_place)
+        self.normalization = normalization
+        self.precision = precision
+        cfg = make_config(lengths, batch, direction, double, in_place, normalization, precision,
+                          measure, device)
+        err = ctypes.create_string_buffer(1024)
+        h = ctypes.c_void_p()
+        self._h = None
+        st = self._lib.syclfft_hip_plan_create_ex(ctypes.byref(cfg), ctypes.byref(h), err, 1024)
+        if st != 0:
+            _raise(st, err)
+        self._h = h
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise SyclfftError("invalid_state", "Cannot execute a moved-from plan")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            self._lib.syclfft_hip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def element_count(self) -> int:
+        n = self.batch
+        for x in self.lengths:
+            n *= x
+        return n
+
+    def scratch_size_bytes(self) -> int:
+        return int(self._lib.syclfft_hip_plan_scratch_bytes(self.handle))
+
+    def describe(self) -> str:
+        n = self._lib.syclfft_hip_plan_describe(self.handle, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        self._lib.syclfft_hip_plan_describe(self.handle, buf, n + 1)
+        return buf.value.decode()
+
+    def launch_count(self) -> int:
+        return int(self._lib.syclfft_hip_plan_launch_count(self.handle))
+
+    def execute_ptr(self, in_ptr: int, out_ptr: int, stream: int = 0,
+                    deps: Iterable[int] = (), done: int = 0) -> None:
+        """Enqueue on a raw hipStream_t with raw hipEvent_t dependencies."""
+        dl = [int(d) for d in deps]
+        arr = (ctypes.c_void_p * max(1, len(dl)))(*dl) if dl else None
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_plan_execute(
+            self.handle, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(stream), arr, len(dl), ctypes.c_void_p(done), err, 1024)
+        if st != 0:
+            _raise(st, err)
+
+    def execute(self, x, y=None, stream=None, deps=(), done=None) -> None:
+        """Execute on torch tensors (complex64 / complex128 on a HIP device).
+
+        execute(inout) for in-place plans, execute(input, output) for out-of-place plans;
+        `stream` is a torch.cuda.Stream (default: the current stream), `deps` torch.cuda.Events
+        to wait for, `done` a torch.cuda.Event recorded at completion."""
+        import torch
+
+        y = x if y is None else y
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        dep_handles = [e.cuda_event for e in deps]
+        self.execute_ptr(x.data_ptr(), y.data_ptr(), s.cuda_stream, dep_handles, 0)
+        if done is not None:
+            done.record(s)

@@ -1,0 +1,135 @@
+// Runs provider plans on the CPU emulation and checks them against the FP64 oracle
+// (TEST INFRASTRUCTURE ONLY).  Usage: emu_driver <case>...  with a case written as
+//   f|d:n0[xn1[xn2]]:batch:dir:norm:inplace      e.g.  f:4096:2:-1:0:0
+// Prints one line per case: "ok <case> relL2=<e>" or "FAIL <case> ..."; exit status 1 on
+// any failure.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "syclfft_hip_provider.h"
+
+extern "C" void oracle_fftn_ld(int rank, const uint64_t *lengths, uint64_t batch, const double *in,
+                               double *out, int sign, int norm);
+
+static bool run_case(const std::string &spec)
+{
+    char prec = 0;
+    char dims[128] = {0};
+    unsigned long long batch = 1;
+    int dir = -1, norm = 0, inplace = 0;
+    if (std::sscanf(spec.c_str(), "%c:%127[0-9x]:%llu:%d:%d:%d", &prec, dims, &batch, &dir, &norm,
+                    &inplace) != 6) {
+        std::printf("FAIL %s bad case spec\n", spec.c_str());
+        return false;
+    }
+    std::vector<uint64_t> lens;
+    for (char *tok = std::strtok(dims, "x"); tok; tok = std::strtok(nullptr, "x")) {
+        lens.push_back(std::strtoull(tok, nullptr, 10));
+    }
+    const bool dbl = prec == 'd';
+    syclfft_hip_plan_config_v1 cfg{};
+    cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
+    cfg.scalar = dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
+    cfg.direction = dir;
+    cfg.rank = static_cast<uint32_t>(lens.size());
+    for (size_t i = 0; i < lens.size(); ++i) {
+        cfg.lengths[i] = lens[i];
+    }
+    cfg.batch_count = batch;
+    cfg.in_place = static_cast<uint32_t>(inplace);
+    cfg.normalization = static_cast<uint32_t>(norm);
+    cfg.device = -1;
+    char err[1024];
+    void *plan = nullptr;
+    const int st = syclfft_hip_plan_create_ex(&cfg, &plan, err, sizeof(err));
+    if (st != 0) {
+        std::printf("FAIL %s create status %d: %s\n", spec.c_str(), st, err);
+        return false;
+    }
+    uint64_t total = batch;
+    for (auto n : lens) {
+        total *= n;
+    }
+    std::vector<double> x(2 * total), ref(2 * total), got(2 * total);
+    std::mt19937_64 rng(20260000 + total);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (auto &v : x) {
+        v = u(rng);
+    }
+    const size_t es = dbl ? 16 : 8;
+    void *din = nullptr, *dout = nullptr;
+    hipMalloc(&din, total * es);
+    if (inplace) {
+        dout = din;
+    } else {
+        hipMalloc(&dout, total * es);
+    }
+    if (dbl) {
+        std::memcpy(din, x.data(), total * 16);
+    } else {
+        float *f = static_cast<float *>(din);
+        for (uint64_t i = 0; i < 2 * total; ++i) {
+            f[i] = static_cast<float>(x[i]);
+            x[i] = f[i];
+        }
+    }
+    const int est = syclfft_hip_plan_execute(plan, din, dout, nullptr, nullptr, 0, nullptr, err, sizeof(err));
+    if (est != 0) {
+        std::printf("FAIL %s execute status %d: %s\n", spec.c_str(), est, err);
+        return false;
+    }
+    if (dbl) {
+        std::memcpy(got.data(), dout, total * 16);
+    } else {
+        const float *f = static_cast<const float *>(dout);
+        for (uint64_t i = 0; i < 2 * total; ++i) {
+            got[i] = f[i];
+        }
+    }
+    oracle_fftn_ld(static_cast<int>(lens.size()), lens.data(), batch, x.data(), ref.data(), dir, norm);
+    double num = 0, den = 0;
+    for (uint64_t i = 0; i < 2 * total; ++i) {
+        num += (got[i] - ref[i]) * (got[i] - ref[i]);
+        den += ref[i] * ref[i];
+    }
+    const double rel = std::sqrt(num / (den > 0 ? den : 1));
+    uint64_t tsize = 1;
+    for (auto n : lens) {
+        tsize *= n;
+    }
+    const double u_ = dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(tsize))) * u_;
+    char desc[4096];
+    syclfft_hip_plan_describe(plan, desc, sizeof(desc));
+    for (char *c = desc; *c; ++c) {
+        if (*c == '\n') {
+            *c = '|';
+        }
+    }
+    const bool ok = rel <= tol && std::isfinite(rel);
+    std::printf("%s %s relL2=%.3e tol=%.1e launches=%u [%s]\n", ok ? "ok" : "FAIL", spec.c_str(), rel,
+                tol, syclfft_hip_plan_launch_count(plan), desc);
+    std::fflush(stdout);
+    syclfft_hip_plan_destroy(plan);
+    hipFree(din);
+    if (!inplace) {
+        hipFree(dout);
+    }
+    return ok;
+}
+
+int main(int argc, char **argv)
+{
+    bool all = true;
+    for (int i = 1; i < argc; ++i) {
+        all = run_case(argv[i]) && all;
+    }
+    return all ? 0 : 1;
+}

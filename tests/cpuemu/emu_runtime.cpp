@@ -1,0 +1,203 @@
+// CPU emulation of the HIP runtime calls the provider makes (TEST INFRASTRUCTURE ONLY).
+#include <hip/hip_runtime.h>
+
+#include <barrier>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "fft_args.h"
+
+thread_local dim3 threadIdx;
+thread_local dim3 blockIdx;
+thread_local dim3 blockDim;
+thread_local dim3 gridDim;
+
+namespace
+{
+std::barrier<> *g_bar = nullptr;
+std::mutex g_mu;
+std::map<uintptr_t, size_t> g_allocs; // base -> size
+hipError_t g_last = hipSuccess;
+int g_device = 0;
+} // namespace
+
+void emu_syncthreads()
+{
+    g_bar->arrive_and_wait();
+}
+
+void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body)
+{
+    const unsigned nt = block.x * block.y * block.z;
+    std::barrier<> bar(nt);
+    g_bar = &bar;
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            threadIdx = dim3(t);
+            blockDim = block;
+            gridDim = grid;
+            for (unsigned b = 0; b < grid.x; ++b) {
+                blockIdx = dim3(b);
+                body();
+                bar.arrive_and_wait();
+            }
+        });
+    }
+    for (auto &x : th) {
+        x.join();
+    }
+    g_bar = nullptr;
+}
+
+hipError_t hipLaunchKernel(const void *fn, dim3 grid, dim3 block, void **args, size_t, hipStream_t)
+{
+    // every kernel launched through this entry takes one sfh_tile_args by value
+    auto kernel = reinterpret_cast<void (*)(sfh_tile_args)>(const_cast<void *>(fn));
+    const sfh_tile_args a = *static_cast<const sfh_tile_args *>(args[0]);
+    if (block.x == 0 || block.x > 1024 || grid.x == 0) {
+        return hipErrorInvalidConfiguration;
+    }
+    emu_launch(grid, block, [&]() { kernel(a); });
+    return hipSuccess;
+}
+
+hipError_t hipGetDeviceCount(int *c)
+{
+    *c = 1;
+    return hipSuccess;
+}
+hipError_t hipGetDevice(int *d)
+{
+    *d = g_device;
+    return hipSuccess;
+}
+hipError_t hipSetDevice(int d)
+{
+    if (d != 0) {
+        return hipErrorInvalidDevice;
+    }
+    g_device = d;
+    return hipSuccess;
+}
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+    std::memset(p, 0, sizeof(*p));
+    std::strcpy(p->name, "cpu-emulated MI355X");
+    std::strcpy(p->gcnArchName, "gfx950:sramecc+:xnack-");
+    p->multiProcessorCount = 256;
+    return hipSuccess;
+}
+hipError_t hipMalloc(void **p, size_t n)
+{
+    *p = std::malloc(n ? n : 1);
+    if (!*p) {
+        return hipErrorOutOfMemory;
+    }
+    std::lock_guard<std::mutex> l(g_mu);
+    g_allocs[reinterpret_cast<uintptr_t>(*p)] = n;
+    return hipSuccess;
+}
+hipError_t hipFree(void *p)
+{
+    if (!p) {
+        return hipSuccess;
+    }
+    std::lock_guard<std::mutex> l(g_mu);
+    g_allocs.erase(reinterpret_cast<uintptr_t>(p));
+    std::free(p);
+    return hipSuccess;
+}
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
+{
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t)
+{
+    std::memset(d, v, n);
+    return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned)
+{
+    *e = reinterpret_cast<hipEvent_t>(new int(0));
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e)
+{
+    delete reinterpret_cast<int *>(e);
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t)
+{
+    return hipSuccess;
+}
+hipError_t hipEventSynchronize(hipEvent_t)
+{
+    return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned)
+{
+    return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t)
+{
+    return hipSuccess;
+}
+static bool find_alloc(const void *p, uintptr_t &base, size_t &size)
+{
+    std::lock_guard<std::mutex> l(g_mu);
+    const auto q = reinterpret_cast<uintptr_t>(p);
+    auto it = g_allocs.upper_bound(q);
+    if (it == g_allocs.begin()) {
+        return false;
+    }
+    --it;
+    if (q >= it->first && q < it->first + std::max<size_t>(it->second, 1)) {
+        base = it->first;
+        size = it->second;
+        return true;
+    }
+    return false;
+}
+hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *p)
+{
+    uintptr_t b;
+    size_t s;
+    std::memset(a, 0, sizeof(*a));
+    if (!find_alloc(p, b, s)) {
+        g_last = hipErrorInvalidValue;
+        return hipErrorInvalidValue;
+    }
+    a->type = hipMemoryTypeDevice;
+    a->device = 0;
+    a->devicePointer = const_cast<void *>(p);
+    return hipSuccess;
+}
+hipError_t hipMemGetAddressRange(hipDeviceptr_t *base, size_t *size, hipDeviceptr_t p)
+{
+    uintptr_t b;
+    size_t s;
+    if (!find_alloc(p, b, s)) {
+        return hipErrorInvalidValue;
+    }
+    *base = reinterpret_cast<void *>(b);
+    *size = s;
+    return hipSuccess;
+}
+hipError_t hipGetLastError()
+{
+    const hipError_t e = g_last;
+    g_last = hipSuccess;
+    return e;
+}
+const char *hipGetErrorString(hipError_t e)
+{
+    return e == hipSuccess ? "success" : "emulated HIP error";
+}

@@ -1,0 +1,109 @@
+"""CPU tests of the C-ABI boundary: the library loads, exports every declared symbol, and
+validates plan configurations the way the reference does before any device work
+(checked_element_count, src/core/provider_loader.cpp:112-132; direction check,
+src/sycl/sycl.cpp:435-438).  No compute call is made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+import syclfft_hip as sh
+
+HEADER = os.path.join(ROOT, "include", "syclfft_hip_provider.h")
+
+
+def declared_symbols():
+    text = "\n".join(l for l in open(HEADER).read().splitlines() if not l.lstrip().startswith("#"))
+    return sorted(set(re.findall(r"SYCLFFT_HIP_API[^;(]*?\b(\w+)\s*\(", text)))
+
+
+def test_header_symbols_match_binding_list():
+    assert declared_symbols() == sorted(sh.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sh.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_provider_table():
+    t = sh.provider_table()
+    assert t.abi_version == sh.ABI_VERSION
+    assert t.name == b"hip_mi355x"
+    for f in ("supports", "create", "destroy", "scratch_bytes", "execute"):
+        assert ctypes.cast(getattr(t, f), ctypes.c_void_p).value
+
+
+def test_config_struct_layout():
+    # fixed C layout: 4 x u32, 3 x u64 lengths, u64 batch, 5 x u32/i32, 3 x u32 reserved
+    assert ctypes.sizeof(sh.PlanConfig) == 16 + 24 + 8 + 20 + 12
+    assert sh.PlanConfig.lengths.offset == 16
+    assert sh.PlanConfig.batch_count.offset == 40
+
+
+def _create_status(cfg):
+    lib = sh.load()
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(256)
+    st = lib.syclfft_hip_plan_create_ex(ctypes.byref(cfg), ctypes.byref(h), err, 256)
+    if h.value:
+        lib.syclfft_hip_plan_destroy(h)
+    return st, err.value.decode()
+
+
+@pytest.mark.parametrize("mutate,status,text", [
+    (lambda c: setattr(c, "rank", 0), 1, "rank"),
+    (lambda c: setattr(c, "rank", 4), 1, "rank"),
+    (lambda c: setattr(c, "batch_count", 0), 1, "batch"),
+    (lambda c: c.lengths.__setitem__(0, 0), 1, "non-zero"),
+    (lambda c: (setattr(c, "rank", 2), c.lengths.__setitem__(0, 2 ** 63), c.lengths.__setitem__(1, 4)), 1, "overflow"),
+    (lambda c: setattr(c, "direction", 0), 1, "direction"),
+    (lambda c: setattr(c, "normalization", 9), 1, "normalization"),
+    (lambda c: setattr(c, "scalar", 3), 4, "scalar"),
+    (lambda c: setattr(c, "abi_version", 2), 8, "ABI"),
+    (lambda c: (setattr(c, "scalar", 2), setattr(c, "precision", 1)), 4, "split"),
+])
+def test_invalid_configs_rejected(mutate, status, text):
+    cfg = sh.make_config([8], 1)
+    mutate(cfg)
+    st, msg = _create_status(cfg)
+    assert st == status, msg
+    assert text.lower() in msg.lower()
+
+
+def test_null_arguments():
+    lib = sh.load()
+    err = ctypes.create_string_buffer(64)
+    assert lib.syclfft_hip_plan_create(None, err, 64) is None
+    assert b"non-null" in err.value
+    # null plan handle
+    st = lib.syclfft_hip_plan_execute(None, None, None, None, None, 0, None, err, 64)
+    assert st == 2
+    lib.syclfft_hip_plan_destroy(None)
+    assert lib.syclfft_hip_plan_scratch_bytes(None) == 0
+
+
+def test_error_buffer_truncation():
+    lib = sh.load()
+    cfg = sh.make_config([8], 1)
+    cfg.rank = 0
+    err = ctypes.create_string_buffer(b"\xff" * 8, 8)
+    h = ctypes.c_void_p()
+    st = lib.syclfft_hip_plan_create_ex(ctypes.byref(cfg), ctypes.byref(h), err, 8)
+    assert st == 1
+    assert err.raw[7] == 0 and len(err.value) == 7
+
+
+def test_no_gpu_reports_unavailable():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    ok, reason = sh.supports(-1)
+    assert not ok and reason
+    with pytest.raises(sh.SyclfftError) as e:
+        sh.Plan([8])
+    assert e.value.code == "provider_unavailable"
