@@ -121,7 +121,7 @@ static bool is_smooth7(uint64_t n)
 }
 
 // Radix program for a 7-smooth length: radix-16 passes for the power-of-two part (then
-// 8/4/2 for the remainder), then 3s, 5s and 7s.
+// 8/4/2 for the remainder), then 9s, 3s, 5s and 7s.
 static std::vector<uint32_t> radix_plan(uint64_t n)
 {
     std::vector<uint32_t> r;
@@ -140,6 +140,10 @@ static std::vector<uint32_t> radix_plan(uint64_t n)
         r.push_back(4);
     } else if (a == 1) {
         r.push_back(2);
+    }
+    while (n % 9 == 0) {
+        r.push_back(9);
+        n /= 9;
     }
     for (uint32_t p : {3u, 5u, 7u}) {
         while (n % p == 0) {

@@ -4,7 +4,7 @@
 // provider (src/sycl/sycl.cpp:87-339).  Where the reference runs one global pass per
 // radix stage plus a gather and a copy-scale pass (14-36 launches per execute), a launch
 // here keeps a 2-D tile (W columns x L points) in LDS and runs the whole length-L FFT
-// as Stockham radix-{2,3,4,5,7,8,16} passes:
+// as Stockham radix-{2,3,4,5,7,8,9,16} passes:
 //   * pass 0 reads its butterfly inputs straight from HBM (coalesced across lanes),
 //   * passes 1..P-1 exchange through LDS (register staging, two barriers per pass),
 //   * the last pass writes its butterfly outputs straight to HBM,
@@ -172,6 +172,16 @@ struct dft<T, 16>
     }
 };
 
+template <typename T>
+struct dft<T, 3>;
+
+// radix 9 as 3 x 3: n = r + 3 m, k = k1 + 3 k2, inner twiddles w9^(r k1)
+template <typename T>
+struct dft9
+{
+    static __device__ __forceinline__ void run(cx<T> *v);
+};
+
 // cos/sin(2 pi p / R) for the odd prime radices
 template <int R>
 struct odd_consts;
@@ -259,6 +269,41 @@ struct dft<T, 5> : dft_odd<T, 5>
 {};
 template <typename T>
 struct dft<T, 7> : dft_odd<T, 7>
+{};
+
+template <typename T>
+__device__ __forceinline__ void dft9<T>::run(cx<T> *v)
+{
+    const T c1 = T(0.76604444311897803520), s1 = T(0.64278760968653932632); // 2 pi / 9
+    const T c2 = T(0.17364817766693034885), s2 = T(0.98480775301220805936); // 4 pi / 9
+    const T c4 = T(-0.93969262078590838405), s4 = T(0.34202014332566873304); // 8 pi / 9
+    cx<T> a[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        cx<T> q[3] = {v[r], v[r + 3], v[r + 6]};
+        dft<T, 3>::run(q);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            a[r][k1] = q[k1];
+        }
+    }
+    a[1][1] = cmul(a[1][1], cx<T>{c1, -s1});
+    a[1][2] = cmul(a[1][2], cx<T>{c2, -s2});
+    a[2][1] = cmul(a[2][1], cx<T>{c2, -s2});
+    a[2][2] = cmul(a[2][2], cx<T>{c4, -s4});
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        cx<T> q[3] = {a[0][k1], a[1][k1], a[2][k1]};
+        dft<T, 3>::run(q);
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) {
+            v[k1 + 3 * k2] = q[k2];
+        }
+    }
+}
+
+template <typename T>
+struct dft<T, 9> : dft9<T>
 {};
 
 
@@ -478,6 +523,9 @@ __device__ __forceinline__ void dispatch_pass(
         break;
     case 8:
         run_pass<T, NT, E, 8, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        break;
+    case 9:
+        run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
         break;
     default:
         run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
