@@ -9,11 +9,12 @@ SFH_DECLARE(f32_128_16)
 SFH_DECLARE(f32_256_16)
 SFH_DECLARE(f32_512_16)
 SFH_DECLARE(f32_1024_16)
+SFH_DECLARE(f32_1024_20)
 SFH_DECLARE(f64_64_16)
 SFH_DECLARE(f64_128_16)
 SFH_DECLARE(f64_256_16)
 SFH_DECLARE(f64_512_16)
-SFH_DECLARE(f64_640_16)
+SFH_DECLARE(f64_512_20)
 #undef SFH_DECLARE
 
 namespace
@@ -28,12 +29,12 @@ struct variant_spec
 const variant_spec k_f32[] = {
     {sfh_kernels_f32_64_16, 64, 16},    {sfh_kernels_f32_128_16, 128, 16},
     {sfh_kernels_f32_256_16, 256, 16},  {sfh_kernels_f32_512_16, 512, 16},
-    {sfh_kernels_f32_1024_16, 1024, 16},
+    {sfh_kernels_f32_1024_16, 1024, 16}, {sfh_kernels_f32_1024_20, 1024, 20},
 };
 const variant_spec k_f64[] = {
     {sfh_kernels_f64_64_16, 64, 16},   {sfh_kernels_f64_128_16, 128, 16},
     {sfh_kernels_f64_256_16, 256, 16}, {sfh_kernels_f64_512_16, 512, 16},
-    {sfh_kernels_f64_640_16, 640, 16},
+    {sfh_kernels_f64_512_20, 512, 20},
 };
 
 constexpr int k_nf32 = sizeof(k_f32) / sizeof(k_f32[0]);
@@ -48,11 +49,13 @@ void init_tables()
         k_f32[i].fill(g_f32[i].fn);
         g_f32[i].nt = k_f32[i].nt;
         g_f32[i].e = k_f32[i].e;
+        g_f32[i].maxr = k_f32[i].e <= 16 ? 16 : 9;
     }
     for (int i = 0; i < k_nf64; ++i) {
         k_f64[i].fill(g_f64[i].fn);
         g_f64[i].nt = k_f64[i].nt;
         g_f64[i].e = k_f64[i].e;
+        g_f64[i].maxr = k_f64[i].e <= 16 ? 16 : 9;
     }
 }
 } // namespace

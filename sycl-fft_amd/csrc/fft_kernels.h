@@ -22,6 +22,7 @@ struct sfh_kernel_variant
     const void *fn[SFH_NUM_MODES]; // __global__ entries per fusion mode
     int nt;                        // threads per workgroup
     int e;                         // elements per thread; LDS capacity = nt * e complex values
+    int maxr;                      // largest radix compiled into the variant (16, or 9 when e > 16)
 };
 
 extern "C" const sfh_kernel_variant *sfh_variants(int is_double, int *count);

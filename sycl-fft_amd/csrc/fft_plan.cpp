@@ -120,9 +120,10 @@ static bool is_smooth7(uint64_t n)
     return n == 1;
 }
 
-// Radix program for a 7-smooth length: radix-16 passes for the power-of-two part (then
-// 8/4/2 for the remainder), then 9s, 3s, 5s and 7s.
-static std::vector<uint32_t> radix_plan(uint64_t n)
+// Radix program for a 7-smooth length: radix-16 passes for the power-of-two part (radix 8
+// when the kernel variant stages more than 16 elements per lane), then 8/4/2 for the
+// remainder, then 9s, 3s, 5s and 7s.
+static std::vector<uint32_t> radix_plan(uint64_t n, uint32_t maxr)
 {
     std::vector<uint32_t> r;
     uint32_t a = 0;
@@ -130,9 +131,13 @@ static std::vector<uint32_t> radix_plan(uint64_t n)
         n /= 2;
         ++a;
     }
-    while (a >= 4) {
+    while (maxr >= 16 && a >= 4) {
         r.push_back(16);
         a -= 4;
+    }
+    while (a >= 3) {
+        r.push_back(8);
+        a -= 3;
     }
     if (a == 3) {
         r.push_back(8);
@@ -332,9 +337,9 @@ static void t4_tables(plan &p, uint64_t N, launch &l)
     l.args.t4bits = bits;
 }
 
-static void set_passes(launch &l, uint32_t L, bool second_fft)
+static void set_passes(launch &l, uint32_t L, bool second_fft, uint32_t maxr)
 {
-    const std::vector<uint32_t> radices = radix_plan(L);
+    const std::vector<uint32_t> radices = radix_plan(L, maxr);
     const uint32_t np = static_cast<uint32_t>(radices.size());
     if (np == 0 || (second_fft ? 2 * np : np) > SFH_MAX_PASSES) {
         throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "unsupported radix program");
@@ -391,12 +396,12 @@ static launch make_launch(plan &p, int mode, const geometry &g, buf_id src, buf_
     l.src = src;
     l.dst = dst;
     sfh_tile_args &a = l.args;
-    set_passes(l, g.L, second_fft);
     l.variant = pick_variant(p, g.L, g.W);
     if (l.variant < 0) {
         throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "no kernel variant fits the tile");
     }
     const sfh_kernel_variant &v = p.variants[l.variant];
+    set_passes(l, g.L, second_fft, static_cast<uint32_t>(v.maxr));
     a.W = g.W;
     a.logW = ilog2(g.W);
     a.LP = g.W > 1 ? g.L + 1 : g.L;

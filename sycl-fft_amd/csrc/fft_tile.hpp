@@ -528,7 +528,11 @@ __device__ __forceinline__ void dispatch_pass(
         run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
         break;
     default:
-        run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        // radix 16 only where at most 16 elements per lane are staged (register budget);
+        // the planner never gives larger-E variants a radix above 9
+        if constexpr (E <= 16) {
+            run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        }
         break;
     }
 }

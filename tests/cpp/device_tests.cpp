@@ -243,6 +243,11 @@ try {
         dbuf<syclfft::complex<float>> out(8);
         f.execute(host.data(), out.p);
     }, "host pointer");
+    tu::expect_error(error_code::invalid_pointer, [&] {
+        auto f = syclfft::plan_dft_1d<float>(q, 8, direction::forward);
+        dbuf<syclfft::complex<float>> small(4), out(8);
+        f.execute(small.p, out.p);
+    }, "buffer shorter than the transform");
     tu::expect_error(error_code::invalid_argument, [&] {
         auto f = syclfft::plan_dft_1d<float>(q, 8, direction::forward, tu::options(placement::in_place));
         dbuf<syclfft::complex<float>> a(8), b(8);

@@ -182,11 +182,9 @@ def test_error_cases():
     with pytest.raises(sh.SyclfftError) as e:
         p.execute_ptr(host.ctypes.data, x.data_ptr())
     assert e.value.code == "invalid_pointer"
-    small = torch.zeros(4, dtype=torch.complex64, device="cuda")
-    big = torch.zeros(8, dtype=torch.complex64, device="cuda")
-    with pytest.raises(sh.SyclfftError) as e:
-        p.execute(small, big)
-    assert e.value.code == "invalid_pointer"
+    # (a buffer shorter than the transform is caught against its HIP allocation; torch's
+    # caching allocator sub-allocates, so that case is tested with raw hipMalloc buffers in
+    # tests/cpp/device_tests.cpp instead)
     ip = sh.Plan([8], in_place=True)
     y = torch.zeros(8, dtype=torch.complex64, device="cuda")
     with pytest.raises(sh.SyclfftError) as e:
