@@ -65,12 +65,15 @@ struct sfh_tile_args
 {
     const void *in;
     void *out;
-    const void *tw;   // W_L^m, m in [0, L): the in-LDS pass twiddles (forward sign)
+    // Twiddle tables staged into LDS at kernel start (tw_entries + t4_entries values):
+    //   pass twiddles W_L^m = hi[m >> tw_bits] * lo[m & (2^tw_bits - 1)]
+    //     lo[x] = W_L^x (x < 2^tw_bits), hi[y] = W_L^(y 2^tw_bits)
+    //   four-step twiddles W_N^m = T0[d0] * T1[d1] * T2[d2], base-2^t4_bits digits of m
+    //     T0[x] = W_N^x, T1[x] = W_N^(x 2^b), T2[y] = W_N^(y 2^2b)
+    const void *twtab;
     const void *pre;  // pre-op table
     const void *mid;  // mid-op table (spectrum with 1/M folded in)
     const void *post; // post-op table
-    const void *t4lo; // W_N^m for m in [0, 2^t4bits)
-    const void *t4hi; // W_N^(m << t4bits)
 
     // tile index g -> (wt, blo, bhi): wt fastest, then blo, then bhi
     uint64_t wcount; // extent of the column dimension (ragged last tile is masked)
@@ -87,6 +90,9 @@ struct sfh_tile_args
     uint32_t tpc;    // threads per column = NT / W
     uint32_t logtpc;
     uint32_t npass;  // passes of FFT #1 (FFT #2 uses the reversed radix order)
+    // passes per radix in the fixed kernel order 16, 8, 4, 2, 9, 3, 5, 7 (FFT #1 runs the
+    // radices in this order, FFT #2 in the reverse order)
+    uint8_t rcount[8];
     uint32_t flags;
     sfh_fastdiv L_div;
     sfh_fastdiv col_div, col_mod;
@@ -94,7 +100,10 @@ struct sfh_tile_args
     uint32_t pre_jmul, pre_cmul, nvalid_in;
     uint32_t mid_kmul, mid_cmul;
     uint32_t post_kmul, post_cmul, nvalid_out;
-    uint32_t t4bits;
+    uint32_t tw_bits, tw_entries; // pass-twiddle tables (lo has 2^tw_bits entries)
+    uint32_t t4_bits, t4_entries; // four-step tables (T0, T1 have 2^t4_bits entries)
+    uint32_t lds_tab;             // LDS element offset of the tables (after the W x LP tile)
+    uint32_t lds_bytes;           // dynamic LDS of the launch
     double scale;
     sfh_pass pass[SFH_MAX_PASSES];
 };

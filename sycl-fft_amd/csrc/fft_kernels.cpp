@@ -43,6 +43,15 @@ sfh_kernel_variant g_f32[k_nf32];
 sfh_kernel_variant g_f64[k_nf64];
 std::once_flag g_once;
 
+// Allow every entry the whole 160 KiB of dynamic LDS a gfx950 workgroup can hold.
+void allow_lds(sfh_kernel_variant &v)
+{
+    for (const void *fn : v.fn) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+    }
+    (void)hipGetLastError();
+}
+
 void init_tables()
 {
     for (int i = 0; i < k_nf32; ++i) {
@@ -50,12 +59,14 @@ void init_tables()
         g_f32[i].nt = k_f32[i].nt;
         g_f32[i].e = k_f32[i].e;
         g_f32[i].maxr = k_f32[i].e <= 16 ? 16 : 9;
+        allow_lds(g_f32[i]);
     }
     for (int i = 0; i < k_nf64; ++i) {
         k_f64[i].fill(g_f64[i].fn);
         g_f64[i].nt = k_f64[i].nt;
         g_f64[i].e = k_f64[i].e;
         g_f64[i].maxr = k_f64[i].e <= 16 ? 16 : 9;
+        allow_lds(g_f64[i]);
     }
 }
 } // namespace
@@ -71,21 +82,43 @@ extern "C" const sfh_kernel_variant *sfh_variants(int is_double, int *count)
     return g_f32;
 }
 
+extern "C" int sfh_device_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cus;
+}
+
+extern "C" int sfh_resident_groups(const sfh_kernel_variant *variant, int mode_index, uint32_t lds_bytes)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, variant->fn[mode_index], variant->nt, lds_bytes)
+        != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
 extern "C" int sfh_launch_tile(
-    const sfh_kernel_variant *variant, int mode_index, const sfh_tile_args *args,
+    const sfh_kernel_variant *variant, int mode_index, const sfh_tile_args *args, uint32_t grid,
     hipStream_t stream)
 {
     if (args->ntiles == 0) {
         return 0;
     }
     // AQL dispatches carry the grid size in work-items as a 32-bit value
-    if (args->ntiles * static_cast<uint64_t>(variant->nt) > 0xffffffffull || mode_index < 0
-        || mode_index >= SFH_NUM_MODES) {
+    if (grid == 0 || grid > args->ntiles || static_cast<uint64_t>(grid) * variant->nt > 0xffffffffull
+        || mode_index < 0 || mode_index >= SFH_NUM_MODES) {
         return static_cast<int>(hipErrorInvalidConfiguration);
     }
     void *kernel_args[] = {const_cast<sfh_tile_args *>(args)};
     const hipError_t st = hipLaunchKernel(
-        variant->fn[mode_index], dim3(static_cast<uint32_t>(args->ntiles)),
-        dim3(static_cast<uint32_t>(variant->nt)), kernel_args, 0, stream);
+        variant->fn[mode_index], dim3(grid), dim3(static_cast<uint32_t>(variant->nt)), kernel_args,
+        args->lds_bytes, stream);
     return static_cast<int>(st);
 }

@@ -14,7 +14,9 @@ enum sfh_mode_index
     SFH_MI_BLUE_A = 3,    // chirp + zero pad, column FFT, twiddle
     SFH_MI_BLUE_B = 4,    // row FFT, spectrum, row FFT, twiddle
     SFH_MI_BLUE_C = 5,    // column FFT, chirp, truncate
-    SFH_NUM_MODES = 6
+    SFH_MI_PLAIN_PF = 6,  // persistent, prefetching form of SFH_MI_PLAIN
+    SFH_MI_T4_PF = 7,     // persistent, prefetching form of SFH_MI_T4
+    SFH_NUM_MODES = 8
 };
 
 struct sfh_kernel_variant
@@ -25,9 +27,15 @@ struct sfh_kernel_variant
     int maxr;                      // largest radix compiled into the variant (16, or 9 when e > 16)
 };
 
+extern "C" int sfh_device_cus();
+
 extern "C" const sfh_kernel_variant *sfh_variants(int is_double, int *count);
+// grid = workgroups launched: args->ntiles for one-tile-per-workgroup entries, at most the
+// resident capacity for the persistent (prefetching) entries
 extern "C" int sfh_launch_tile(
-    const sfh_kernel_variant *variant, int mode_index, const sfh_tile_args *args,
+    const sfh_kernel_variant *variant, int mode_index, const sfh_tile_args *args, uint32_t grid,
     hipStream_t stream);
+// workgroups of one entry resident per CU at the given dynamic LDS (0 if unknown)
+extern "C" int sfh_resident_groups(const sfh_kernel_variant *variant, int mode_index, uint32_t lds_bytes);
 extern "C" int sfh_launch_copy_scale(
     const void *in, void *out, uint64_t count, int is_double, double scale, hipStream_t stream);

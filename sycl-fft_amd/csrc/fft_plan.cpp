@@ -200,8 +200,10 @@ struct launch
     int mode{};
     sfh_tile_args args{};
     buf_id src{BUF_IN}, dst{BUF_OUT};
+    int64_t src_off{0}, dst_off{0}; // element offsets into the source / destination buffer
+    uint32_t grid{};                // workgroups launched
     // table offsets into the plan's table block (patched into args at plan time)
-    int64_t tw_off{-1}, pre_off{-1}, mid_off{-1}, post_off{-1}, t4lo_off{-1}, t4hi_off{-1};
+    int64_t tw_off{-1}, pre_off{-1}, mid_off{-1}, post_off{-1};
     std::string desc;
 };
 
@@ -245,13 +247,52 @@ static int max_cap(const plan &p)
     return best;
 }
 
-static int slack_of(const plan &p, int cap)
+// Sizes of the LDS twiddle tables of a launch: a two-level table for the pass twiddles of
+// length L (2^tw_bits + ceil(L / 2^tw_bits) entries, the split minimizing the count) and,
+// when the launch applies four-step twiddles of length N, a three-level table
+// (2 * 2^t4_bits + ceil(N / 2^(2 t4_bits)) entries).
+struct tw_layout
 {
-    return (static_cast<size_t>(cap + 64) * p.esize > 163840) ? 0 : 64;
+    uint32_t tw_bits{}, tw_entries{}, t4_bits{}, t4_entries{};
+};
+
+static tw_layout twiddle_layout(uint64_t L, uint64_t N)
+{
+    tw_layout t;
+    uint64_t best = UINT64_MAX;
+    for (uint32_t b = 0; b <= 16; ++b) {
+        const uint64_t K = uint64_t(1) << b;
+        const uint64_t e = K + (L + K - 1) / K;
+        if (e < best) {
+            best = e;
+            t.tw_bits = b;
+        }
+    }
+    t.tw_entries = static_cast<uint32_t>(best);
+    if (N) {
+        uint32_t b = (ilog2(N) + 2) / 3;
+        b = std::max<uint32_t>(b, 1);
+        const uint64_t K = uint64_t(1) << b;
+        t.t4_bits = b;
+        t.t4_entries = static_cast<uint32_t>(2 * K + (N + K * K - 1) / (K * K));
+    }
+    return t;
 }
 
-// Smallest variant holding W columns of length L (pitch L+1 when W > 1).
-static int pick_variant(const plan &p, uint32_t L, uint32_t W)
+static uint64_t lds_bytes_of(const plan &p, uint32_t L, uint32_t W, const tw_layout &t, uint32_t *tab_off)
+{
+    const uint64_t pitch = W > 1 ? L + 1 : L;
+    const uint64_t tab = (pitch * W + 1) & ~uint64_t(1);
+    if (tab_off) {
+        *tab_off = static_cast<uint32_t>(tab);
+    }
+    return (tab + t.tw_entries + t.t4_entries) * p.esize;
+}
+
+static constexpr uint64_t k_lds_limit = 163840; // 160 KiB of LDS per workgroup on gfx950
+
+// Smallest variant holding W columns of length L (pitch L+1 when W > 1) plus the tables.
+static int pick_variant(const plan &p, uint32_t L, uint32_t W, const tw_layout &t)
 {
     int best = -1;
     for (int i = 0; i < p.nvariants; ++i) {
@@ -259,11 +300,10 @@ static int pick_variant(const plan &p, uint32_t L, uint32_t W)
             continue;
         }
         const int cap = p.variants[i].nt * p.variants[i].e;
-        const uint64_t pitch = W > 1 ? L + 1 : L;
         if (static_cast<uint64_t>(L) * W > static_cast<uint64_t>(cap)) {
             continue;
         }
-        if (pitch * W > static_cast<uint64_t>(cap + slack_of(p, cap))) {
+        if (lds_bytes_of(p, L, W, t, nullptr) > k_lds_limit) {
             continue;
         }
         if (static_cast<uint32_t>(p.variants[i].nt) < W) {
@@ -277,6 +317,11 @@ static int pick_variant(const plan &p, uint32_t L, uint32_t W)
         }
     }
     return best;
+}
+
+static int pick_variant(const plan &p, uint32_t L, uint32_t W)
+{
+    return pick_variant(p, L, W, twiddle_layout(L, 0));
 }
 
 template <class T>
@@ -302,39 +347,35 @@ static int64_t table(plan &p, const std::string &key, const std::vector<cxl> &va
     return p.dbl ? add_table<double>(p, key, values) : add_table<float>(p, key, values);
 }
 
-static int64_t twiddle_table(plan &p, uint64_t L)
+// The table block a launch stages into LDS (layout of sfh_tile_args::twtab).
+static int64_t twiddle_tables(plan &p, uint64_t L, uint64_t N, const tw_layout &t)
 {
-    const std::string key = "tw" + std::to_string(L);
+    const std::string key = "tw" + std::to_string(L) + "_" + std::to_string(N);
     if (p.table_cache.count(key)) {
         return p.table_cache[key];
     }
-    std::vector<cxl> v(L);
-    for (uint64_t m = 0; m < L; ++m) {
-        v[m] = root_ld(m, L);
+    std::vector<cxl> v;
+    v.reserve(t.tw_entries + t.t4_entries);
+    const uint64_t K = uint64_t(1) << t.tw_bits;
+    for (uint64_t x = 0; x < K; ++x) {
+        v.push_back(root_ld(x, L));
+    }
+    for (uint64_t y = 0; y < t.tw_entries - K; ++y) {
+        v.push_back(root_ld(y * K, L));
+    }
+    if (N) {
+        const uint64_t K4 = uint64_t(1) << t.t4_bits;
+        for (uint64_t x = 0; x < K4; ++x) {
+            v.push_back(root_ld(x, N));
+        }
+        for (uint64_t x = 0; x < K4; ++x) {
+            v.push_back(root_ld(x * K4, N));
+        }
+        for (uint64_t y = 0; y < t.t4_entries - 2 * K4; ++y) {
+            v.push_back(root_ld(y * K4 * K4, N));
+        }
     }
     return table(p, key, v);
-}
-
-static void t4_tables(plan &p, uint64_t N, launch &l)
-{
-    const uint32_t bits = (ilog2(N) + 1) / 2;
-    const uint64_t lo_n = uint64_t(1) << bits;
-    const uint64_t hi_n = (N + lo_n - 1) / lo_n;
-    const std::string klo = "t4lo" + std::to_string(N), khi = "t4hi" + std::to_string(N);
-    if (!p.table_cache.count(klo)) {
-        std::vector<cxl> lo(lo_n), hi(hi_n);
-        for (uint64_t m = 0; m < lo_n; ++m) {
-            lo[m] = root_ld(m, N);
-        }
-        for (uint64_t m = 0; m < hi_n; ++m) {
-            hi[m] = root_ld(m * lo_n, N);
-        }
-        table(p, klo, lo);
-        table(p, khi, hi);
-    }
-    l.t4lo_off = p.table_cache[klo];
-    l.t4hi_off = p.table_cache[khi];
-    l.args.t4bits = bits;
 }
 
 static void set_passes(launch &l, uint32_t L, bool second_fft, uint32_t maxr)
@@ -356,6 +397,23 @@ static void set_passes(launch &l, uint32_t L, bool second_fft, uint32_t maxr)
         }
     };
     fill(0, radices);
+    // per-radix pass counts in the kernel's fixed order; radix_plan emits that order
+    static const uint32_t order[8] = {16, 8, 4, 2, 9, 3, 5, 7};
+    uint32_t pos = 0;
+    for (int k = 0; k < 8; ++k) {
+        uint32_t cnt = 0;
+        while (pos < radices.size() && radices[pos] == order[k]) {
+            ++cnt;
+            ++pos;
+        }
+        l.args.rcount[k] = static_cast<uint8_t>(cnt);
+    }
+    if (pos != radices.size()) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "radix program out of kernel order");
+    }
+    if (second_fft && np < 2) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "Bluestein length needs two radix passes");
+    }
     if (second_fft) {
         std::vector<uint32_t> rev(radices.rbegin(), radices.rend());
         fill(np, rev);
@@ -387,19 +445,24 @@ static void check_span(const geometry &g)
     }
 }
 
-static launch make_launch(plan &p, int mode, const geometry &g, buf_id src, buf_id dst,
-                          bool second_fft, const std::string &what)
+static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
+                          bool second_fft, const std::string &what, uint64_t t4n = 0)
 {
-    check_span(g);
     launch l;
     l.mode = mode;
     l.src = src;
     l.dst = dst;
     sfh_tile_args &a = l.args;
-    l.variant = pick_variant(p, g.L, g.W);
+    const tw_layout tl = twiddle_layout(g.L, t4n);
+    l.variant = pick_variant(p, g.L, g.W, tl);
+    while (l.variant < 0 && g.W > 1) {
+        g.W /= 2; // the tables did not fit beside a tile this wide
+        l.variant = pick_variant(p, g.L, g.W, tl);
+    }
     if (l.variant < 0) {
         throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "no kernel variant fits the tile");
     }
+    check_span(g);
     const sfh_kernel_variant &v = p.variants[l.variant];
     set_passes(l, g.L, second_fft, static_cast<uint32_t>(v.maxr));
     a.W = g.W;
@@ -435,11 +498,17 @@ static launch make_launch(plan &p, int mode, const geometry &g, buf_id src, buf_
     } else if (out_wc) {
         a.flags |= SFH_STORE_LDS;
     }
-    l.tw_off = twiddle_table(p, g.L);
+    a.tw_bits = tl.tw_bits;
+    a.tw_entries = tl.tw_entries;
+    a.t4_bits = tl.t4_bits;
+    a.t4_entries = tl.t4_entries;
+    a.lds_bytes = static_cast<uint32_t>(lds_bytes_of(p, g.L, g.W, tl, &a.lds_tab));
+    l.tw_off = twiddle_tables(p, g.L, t4n, tl);
+    l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
     char buf[256];
     std::snprintf(
-        buf, sizeof(buf), "%s L=%u W=%u tiles=%llu nt=%d e=%d mode=%d flags=0x%x", what.c_str(),
-        g.L, g.W, static_cast<unsigned long long>(a.ntiles), v.nt, v.e, mode, a.flags);
+        buf, sizeof(buf), "%s L=%u W=%u tiles=%llu nt=%d e=%d mode=%d flags=0x%x lds=%u", what.c_str(),
+        g.L, g.W, static_cast<unsigned long long>(a.ntiles), v.nt, v.e, mode, a.flags, a.lds_bytes);
     l.desc = buf;
     return l;
 }
@@ -502,6 +571,28 @@ static uint32_t min_cols(const plan &p)
 }
 
 // One axis of extent n with `outer` lines before and `inner` elements after it.
+static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, uint64_t n1, uint64_t n2,
+                          buf_id src, buf_id dst, int64_t off);
+static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_id src, buf_id dst);
+
+// Lines of an axis processed per launch group when the axis needs scratch between
+// launches: with SYCLFFT_HIP_CHUNK_MB > 0 the group's intermediate (that many MiB) can stay
+// in the 256 MiB Infinity Cache between the launches that write and read it.  Off by
+// default (one launch group per axis).
+static uint64_t chunk_lines(const plan &p, uint64_t outer, uint64_t line_elems)
+{
+    uint64_t mb = 0; // measured: the extra launch boundaries cost more than the cache reuse gains
+    if (const char *e = std::getenv("SYCLFFT_HIP_CHUNK_MB")) {
+        mb = std::strtoull(e, nullptr, 10);
+    }
+    if (mb == 0) {
+        return outer;
+    }
+    const uint64_t bytes = mb << 20;
+    const uint64_t per = std::max<uint64_t>(1, bytes / std::max<uint64_t>(1, line_elems * p.esize));
+    return std::min(per, outer);
+}
+
 static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_id src, buf_id dst)
 {
     if (n == 1) {
@@ -539,7 +630,22 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
             }
             wmin /= 2;
         }
-        p.scratch_elems = std::max(p.scratch_elems, p.total);
+        const uint64_t chunk = chunk_lines(p, outer, n * inner);
+        p.scratch_elems = std::max(p.scratch_elems, chunk * n * inner);
+        for (uint64_t o0 = 0; o0 < outer; o0 += chunk) {
+            const uint64_t cnt = std::min(chunk, outer - o0);
+            const int64_t off = static_cast<int64_t>(o0 * n * inner);
+            plan_fourstep(p, cnt, n, inner, n1, n2, src, dst, off);
+        }
+        return;
+    }
+    plan_bluestein(p, outer, n, inner, src, dst);
+}
+
+// Two launches over `outer` lines starting at element `off` of src/dst (scratch from 0).
+static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, uint64_t n1, uint64_t n2,
+                          buf_id src, buf_id dst, int64_t off)
+{
         {
             // pass A: FFT over j1 (n1 points, stride n2*inner) for every (outer, j2, inner),
             // twiddle W_n^(j2 k1), in place positions into scratch.
@@ -553,8 +659,8 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
             g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
             g.col_div = inner;
             g.col_mod = n2;
-            launch l = make_launch(p, SFH_MI_T4, g, src, BUF_SCRATCH, false, "fourstep-A");
-            t4_tables(p, n, l);
+            launch l = make_launch(p, SFH_MI_T4, g, src, BUF_SCRATCH, false, "fourstep-A", n);
+            l.src_off = off;
             p.launches.push_back(l);
         }
         {
@@ -582,11 +688,14 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
                 g.out_blo = static_cast<int64_t>(inner);
                 g.in_bhi = g.out_bhi = static_cast<int64_t>(n * inner);
             }
-            p.launches.push_back(make_launch(p, SFH_MI_PLAIN, g, BUF_SCRATCH, dst, false, "fourstep-B"));
+            launch l = make_launch(p, SFH_MI_PLAIN, g, BUF_SCRATCH, dst, false, "fourstep-B");
+            l.dst_off = off;
+            p.launches.push_back(l);
         }
-        return;
-    }
+}
 
+static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_id src, buf_id dst)
+{
     // ---------------- Bluestein ----------------
     // chirp c[j] = exp(-i pi j^2/n); b[m] = conj(c[m]) for |m| < n wrapped mod M;
     // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
@@ -671,8 +780,20 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
         return;
     }
 
-    // three launches over a (outer, M, inner) scratch, M = m1 * m2
-    p.scratch_elems = std::max(p.scratch_elems, outer * M * inner);
+    // three launches over a (outer, M, inner) scratch, M = m1 * m2, per group of lines
+    std::vector<cxl> spec_t(M);
+    for (uint64_t k1 = 0; k1 < m1; ++k1) {
+        for (uint64_t k2 = 0; k2 < m2; ++k2) {
+            spec_t[k1 * m2 + k2] = spec[k1 + m1 * k2];
+        }
+    }
+    const int64_t spect_off = table(p, "spect" + nkey, spec_t);
+    const uint64_t all = outer;
+    const uint64_t chunk = chunk_lines(p, all, M * inner);
+    p.scratch_elems = std::max(p.scratch_elems, chunk * M * inner);
+    for (uint64_t o0 = 0; o0 < all; o0 += chunk) {
+    outer = std::min(chunk, all - o0);
+    const int64_t off = static_cast<int64_t>(o0 * n * inner);
     {
         // A: a[j] = x[j] c[j] (j < n), column FFT over j1 of j = j1 m2 + j2, twiddle W_M^(j2 k1)
         geometry g;
@@ -688,22 +809,16 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
         g.out_blo = static_cast<int64_t>(M * inner);
         g.col_div = inner;
         g.col_mod = m2;
-        launch l = make_launch(p, SFH_MI_BLUE_A, g, src, BUF_SCRATCH, false, "bluestein-A");
+        launch l = make_launch(p, SFH_MI_BLUE_A, g, src, BUF_SCRATCH, false, "bluestein-A", M);
         l.args.pre_jmul = static_cast<uint32_t>(m2);
         l.args.pre_cmul = 1;
         l.args.nvalid_in = static_cast<uint32_t>(n);
         l.pre_off = chirp_off;
-        t4_tables(p, M, l);
+        l.src_off = off;
         p.launches.push_back(l);
     }
     {
         // B: row k1: FFT over j2, * spectrum (stored transposed), conj, FFT, twiddle W_M^(k1 j2)
-        std::vector<cxl> spec_t(M);
-        for (uint64_t k1 = 0; k1 < m1; ++k1) {
-            for (uint64_t k2 = 0; k2 < m2; ++k2) {
-                spec_t[k1 * m2 + k2] = spec[k1 + m1 * k2];
-            }
-        }
         geometry g;
         g.L = static_cast<uint32_t>(m2);
         if (inner == 1) {
@@ -728,11 +843,10 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
             g.col_mod = 1;
             g.col_bmul = 1;
         }
-        launch l = make_launch(p, SFH_MI_BLUE_B, g, BUF_SCRATCH, BUF_SCRATCH, true, "bluestein-B");
+        launch l = make_launch(p, SFH_MI_BLUE_B, g, BUF_SCRATCH, BUF_SCRATCH, true, "bluestein-B", M);
         l.args.mid_kmul = 1;
         l.args.mid_cmul = static_cast<uint32_t>(m2);
-        l.mid_off = table(p, "spect" + nkey, spec_t);
-        t4_tables(p, M, l);
+        l.mid_off = spect_off;
         p.launches.push_back(l);
     }
     {
@@ -756,7 +870,9 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
         l.args.nvalid_out = static_cast<uint32_t>(n);
         l.args.flags |= SFH_CONJ_POST;
         l.post_off = chirp_off;
+        l.dst_off = off;
         p.launches.push_back(l);
+    }
     }
 }
 
@@ -923,6 +1039,8 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
     p->scale = norm_scale(cfg->normalization, cfg->direction, p->transform);
 
     // axis loop in the reference order (src/sycl/sycl.cpp:692-767)
+    size_t first_begin = 0, first_end = 0, last_begin = 0;
+    bool any = false;
     for (uint32_t axis = 0; axis < cfg->rank; ++axis) {
         uint64_t inner = 1;
         for (uint32_t k = axis + 1; k < cfg->rank; ++k) {
@@ -931,22 +1049,67 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
         const uint64_t n = cfg->lengths[axis];
         const uint64_t outer = p->total / (n * inner);
         const buf_id src = p->launches.empty() ? BUF_IN : BUF_OUT;
+        const size_t begin = p->launches.size();
         plan_axis(*p, outer, n, inner, src, BUF_OUT);
+        if (p->launches.size() > begin) {
+            if (!any) {
+                first_begin = begin;
+                first_end = p->launches.size();
+                any = true;
+            }
+            last_begin = begin;
+        }
     }
     if (p->launches.empty()) {
         p->copy_only = 1; // every length is 1: out = scale * in
     } else {
-        if (cfg->direction == 1) {
-            p->launches.front().args.flags |= SFH_CONJ_IN;
-            p->launches.back().args.flags |= SFH_CONJ_OUT;
+        // backward = conj(FFT(conj x)): conjugate where the plan input is read and where the
+        // final output is written; the scale goes on every final write
+        for (size_t i = first_begin; i < first_end; ++i) {
+            if (p->launches[i].src == BUF_IN && cfg->direction == 1) {
+                p->launches[i].args.flags |= SFH_CONJ_IN;
+            }
         }
-        if (p->scale != 1.0) {
-            p->launches.back().args.flags |= SFH_SCALE;
-            p->launches.back().args.scale = p->scale;
+        for (size_t i = last_begin; i < p->launches.size(); ++i) {
+            launch &l = p->launches[i];
+            if (l.dst != BUF_OUT) {
+                continue;
+            }
+            if (cfg->direction == 1) {
+                l.args.flags |= SFH_CONJ_OUT;
+            }
+            if (p->scale != 1.0) {
+                l.args.flags |= SFH_SCALE;
+                l.args.scale = p->scale;
+            }
         }
     }
 
     device_guard guard(dev);
+    // Persistent prefetching form for plain and four-step-A launches with more tiles than
+    // resident workgroups: SYCLFFT_HIP_PREFETCH=1 enables it (measured slower than one tile
+    // per workgroup at full occupancy on the batched shapes, so off by default).
+    {
+        const char *pe = std::getenv("SYCLFFT_HIP_PREFETCH");
+        const bool allow = pe && std::atoi(pe) != 0;
+        const int cus = sfh_device_cus();
+        for (auto &l : p->launches) {
+            if (!allow || cus <= 0 || (l.args.flags & SFH_LOAD_LDS)) {
+                continue;
+            }
+            const int pf_mode = l.mode == SFH_MI_PLAIN ? SFH_MI_PLAIN_PF : (l.mode == SFH_MI_T4 ? SFH_MI_T4_PF : -1);
+            if (pf_mode < 0) {
+                continue;
+            }
+            const int per_cu = sfh_resident_groups(&p->variants[l.variant], pf_mode, l.args.lds_bytes);
+            const uint64_t cap = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(cus);
+            if (per_cu > 0 && l.args.ntiles > cap) {
+                l.mode = pf_mode;
+                l.grid = static_cast<uint32_t>(cap);
+                l.desc += " persistent grid=" + std::to_string(cap);
+            }
+        }
+    }
     if (!p->host_tables.empty()) {
         hip_check(hipMalloc(&p->tables, p->host_tables.size()), "hipMalloc(tables)");
         hip_check(
@@ -961,12 +1124,10 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
         return off < 0 ? nullptr : static_cast<const unsigned char *>(p->tables) + off;
     };
     for (auto &l : p->launches) {
-        l.args.tw = ptr(l.tw_off);
+        l.args.twtab = ptr(l.tw_off);
         l.args.pre = ptr(l.pre_off);
         l.args.mid = ptr(l.mid_off);
         l.args.post = ptr(l.post_off);
-        l.args.t4lo = ptr(l.t4lo_off);
-        l.args.t4hi = ptr(l.t4hi_off);
     }
     p->host_tables.clear();
     p->host_tables.shrink_to_fit();
@@ -1055,10 +1216,10 @@ static void execute_plan(
     void *bufs[3] = {const_cast<void *>(in), out, p.scratch};
     for (auto &l : p.launches) {
         sfh_tile_args a = l.args;
-        a.in = bufs[l.src];
-        a.out = bufs[l.dst];
+        a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
+        a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
         hip_check(
-            static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, stream)),
+            static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, l.grid, stream)),
             "tile kernel launch");
     }
     hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");

@@ -23,8 +23,12 @@
 
 #if defined(__HIP__)
 #define SFH_WAVES_PER_EU(...) __attribute__((amdgpu_waves_per_eu(__VA_ARGS__)))
+#define SFH_DYNAMIC_LDS(name) extern __shared__ __attribute__((aligned(16))) unsigned char name[]
+#define SFH_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
 #else
+#define SFH_SCHED_BARRIER()
 #define SFH_WAVES_PER_EU(...)
+#define SFH_DYNAMIC_LDS(name) unsigned char *name = emu_dynamic_lds()
 #endif
 
 namespace sfh
@@ -324,6 +328,9 @@ struct tile_ctx
     uint32_t lt;       // this lane's position inside its column
     uint32_t blo;
     uint32_t col;      // logical column of this lane's column
+    const cx<T> *tw;   // pass-twiddle tables in LDS
+    const cx<T> *t4;   // four-step tables in LDS
+    bool wvalid;       // this lane's column exists in the tile
 };
 
 __device__ __forceinline__ uint32_t col_of(const sfh_tile_args &a, uint32_t w0, uint32_t blo, uint32_t w)
@@ -355,16 +362,60 @@ __device__ __forceinline__ cx<T> load_one(const sfh_tile_args &a, const cx<T> *c
     }
 }
 
-template <typename T, int MODE>
+// W_L^m from the two-level pass table in LDS
+template <typename T>
+__device__ __forceinline__ cx<T> pass_twiddle(const cx<T> *tab, uint32_t bits, uint32_t m)
+{
+    return cmul(tab[(1u << bits) + (m >> bits)], tab[m & ((1u << bits) - 1u)]);
+}
+
+// W_N^m from the three-level four-step table in LDS
+template <typename T>
+__device__ __forceinline__ cx<T> t4_twiddle(const sfh_tile_args &a, const cx<T> *tab, uint32_t m)
+{
+    const uint32_t b = a.t4_bits, mask = (1u << b) - 1u;
+    const cx<T> w0 = tab[m & mask];
+    const cx<T> w1 = tab[(1u << b) + ((m >> b) & mask)];
+    const cx<T> w2 = tab[(2u << b) + (m >> (2 * b))];
+    return cmul(cmul(w2, w1), w0);
+}
+
+// w[t] = W^(e t) for t < R from four table lookups (t = 1, 2, 4, 8) and at most three
+// products each: lk(x) returns W^x for x < the table length.
+template <typename T, int R, class Lookup>
+__device__ __forceinline__ void twiddle_powers(cx<T> *w, uint32_t e, Lookup lk)
+{
+    w[0] = {T(1), T(0)};
+    if constexpr (R > 1) {
+        w[1] = lk(e);
+    }
+    if constexpr (R > 2) {
+        w[2] = lk(2 * e);
+    }
+    if constexpr (R > 4) {
+        w[4] = lk(4 * e);
+    }
+    if constexpr (R > 8) {
+        w[8] = lk(8 * e);
+    }
+#pragma unroll
+    for (int t = 3; t < R; ++t) {
+        if (t == 4 || t == 8) {
+            continue;
+        }
+        const int hb = t >= 8 ? 8 : (t >= 4 ? 4 : 2);
+        w[t] = cmul(w[hb], w[t - hb]);
+    }
+}
+
+// T4 = 1: apply the four-step twiddle W_N^(col k) inside (else the caller already did).
+template <typename T, int MODE, int T4 = 1>
 __device__ __forceinline__ void store_one(
-    const sfh_tile_args &a, cx<T> *colp, uint32_t k, uint32_t col, cx<T> v)
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *colp, uint32_t k, uint32_t col, cx<T> v)
 {
     const uint32_t f = a.flags;
-    if constexpr (MODE & SFH_M_T4) {
-        const uint32_t m = col * k;
-        const cx<T> lo = static_cast<const cx<T> *>(a.t4lo)[m & ((1u << a.t4bits) - 1u)];
-        const cx<T> hi = static_cast<const cx<T> *>(a.t4hi)[m >> a.t4bits];
-        v = cmul(v, cmul(hi, lo));
+    if constexpr ((MODE & SFH_M_T4) && T4) {
+        v = cmul(v, t4_twiddle(a, c.t4, col * k));
     }
     if (f & SFH_CONJ_POST) {
         v.y = -v.y;
@@ -398,7 +449,25 @@ __device__ __forceinline__ void store_one(
 enum
 {
     IN_GLOBAL = 0,
-    IN_LDS = 1
+    IN_LDS = 1,
+    IN_REGS = 2 // pass 0 inputs already in the prefetch registers (loaded by prefetch_pass0)
+};
+
+// Largest register footprint (butterflies per lane x radix) of any pass radix.
+template <int E>
+struct prefetch_size
+{
+    static constexpr int nbr(int r)
+    {
+        return (E + r - 1) / r * r;
+    }
+    static constexpr int max2(int x, int y)
+    {
+        return x > y ? x : y;
+    }
+    static constexpr int value = max2(
+        max2(max2(nbr(2), nbr(3)), max2(nbr(4), nbr(5))),
+        max2(max2(nbr(7), nbr(8)), max2(nbr(9), E <= 16 ? nbr(16) : 0)));
 };
 enum
 {
@@ -407,14 +476,32 @@ enum
     OUT_MID = 2
 };
 
+// Load the inputs of pass 0 (radix R) of the lane's butterflies into pf (with pre-ops).
+template <typename T, int E, int R, int MODE>
+__device__ __forceinline__ void prefetch_pass0(
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *pf, bool wvalid)
+{
+    constexpr int NB = (E + R - 1) / R;
+    const uint32_t m = a.L / R;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+        if (wvalid && i < m) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                pf[q * R + t] = load_one<T, MODE>(a, c.src, i + t * m, c.col);
+            }
+        }
+    }
+}
+
 template <typename T, int NT, int E, int R, int MODE>
 __device__ __forceinline__ void run_pass(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
-    int out_mode, bool wvalid)
+    int out_mode, bool wvalid, const cx<T> *pf)
 {
     constexpr int NB = (E + R - 1) / R;
     const uint32_t m = a.L / R; // butterflies per column
-    const cx<T> *tw = static_cast<const cx<T> *>(a.tw);
     cx<T> *ldsc = lds + c.w * a.LP;
     cx<T> v[NB][R];
 
@@ -422,7 +509,12 @@ __device__ __forceinline__ void run_pass(
     for (int q = 0; q < NB; ++q) {
         const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
         if (wvalid && i < m) {
-            if (in_mode == IN_GLOBAL) {
+            if (in_mode == IN_REGS) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    v[q][t] = pf[q * R + t];
+                }
+            } else if (in_mode == IN_GLOBAL) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     v[q][t] = load_one<T, MODE>(a, c.src, i + t * m, c.col);
@@ -435,14 +527,19 @@ __device__ __forceinline__ void run_pass(
                 if (p.ns > 1) {
                     const uint32_t kq = fdiv(i, p.ns_div);
                     const uint32_t step = (i - kq * p.ns) * p.twstride;
+                    cx<T> w[R];
+                    twiddle_powers<T, R>(w, step, [&](uint32_t x) { return pass_twiddle(c.tw, a.tw_bits, x); });
 #pragma unroll
                     for (int t = 1; t < R; ++t) {
-                        v[q][t] = cmul(v[q][t], tw[step * t]);
+                        v[q][t] = cmul(v[q][t], w[t]);
                     }
                 }
             }
             dft<T, R>::run(v[q]);
         }
+        // keep the scheduler from hoisting the next butterfly's loads and twiddles above this
+        // one: bounds the live registers to one butterfly's working set plus the outputs
+        SFH_SCHED_BARRIER();
     }
 
     if (out_mode == OUT_GLOBAL) {
@@ -450,9 +547,21 @@ __device__ __forceinline__ void run_pass(
         for (int q = 0; q < NB; ++q) {
             const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
             if (wvalid && i < m) {
+                if constexpr (MODE & SFH_M_T4) {
+                    // W_N^(col (i + t m)) = W_N^(col i) (W_N^(col m))^t
+                    const auto lk = [&](uint32_t x) { return t4_twiddle(a, c.t4, x); };
+                    const cx<T> base = lk(c.col * i);
+                    cx<T> w[R];
+                    twiddle_powers<T, R>(w, c.col * m, lk);
 #pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    store_one<T, MODE>(a, c.dst, i + t * m, c.col, v[q][t]);
+                    for (int t = 0; t < R; ++t) {
+                        store_one<T, MODE, 0>(a, c, c.dst, i + t * m, c.col, cmul(v[q][t], cmul(base, w[t])));
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        store_one<T, MODE>(a, c, c.dst, i + t * m, c.col, v[q][t]);
+                    }
                 }
             }
         }
@@ -500,40 +609,152 @@ __device__ __forceinline__ void run_pass(
     __syncthreads();
 }
 
+template <typename T, int E, int MODE>
+__device__ __forceinline__ void dispatch_prefetch(
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *pf, bool wvalid)
+{
+    switch (a.pass[0].radix) {
+    case 2:
+        prefetch_pass0<T, E, 2, MODE>(a, c, pf, wvalid);
+        break;
+    case 3:
+        prefetch_pass0<T, E, 3, MODE>(a, c, pf, wvalid);
+        break;
+    case 4:
+        prefetch_pass0<T, E, 4, MODE>(a, c, pf, wvalid);
+        break;
+    case 5:
+        prefetch_pass0<T, E, 5, MODE>(a, c, pf, wvalid);
+        break;
+    case 7:
+        prefetch_pass0<T, E, 7, MODE>(a, c, pf, wvalid);
+        break;
+    case 8:
+        prefetch_pass0<T, E, 8, MODE>(a, c, pf, wvalid);
+        break;
+    case 9:
+        prefetch_pass0<T, E, 9, MODE>(a, c, pf, wvalid);
+        break;
+    default:
+        if constexpr (E <= 16) {
+            prefetch_pass0<T, E, 16, MODE>(a, c, pf, wvalid);
+        }
+        break;
+    }
+}
+
 template <typename T, int NT, int E, int MODE>
 __device__ __forceinline__ void dispatch_pass(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
-    int out_mode, bool wvalid)
+    int out_mode, bool wvalid, const cx<T> *pf)
 {
     switch (p.radix) {
     case 2:
-        run_pass<T, NT, E, 2, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 2, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 3:
-        run_pass<T, NT, E, 3, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 3, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 4:
-        run_pass<T, NT, E, 4, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 4, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 5:
-        run_pass<T, NT, E, 5, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 5, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 7:
-        run_pass<T, NT, E, 7, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 7, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 8:
-        run_pass<T, NT, E, 8, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 8, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     case 9:
-        run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+        run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         break;
     default:
         // radix 16 only where at most 16 elements per lane are staged (register budget);
         // the planner never gives larger-E variants a radix above 9
         if constexpr (E <= 16) {
-            run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid);
+            run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
         }
         break;
+    }
+}
+
+// Out mode of fused-program step s (FFT #1 passes 0..np-1, then FFT #2 passes 1..np-1).
+__device__ __forceinline__ int out_mode_of(uint32_t s, uint32_t total, uint32_t np, bool two, uint32_t flags)
+{
+    if (s + 1 == total) {
+        return (flags & SFH_STORE_LDS) ? OUT_LDS : OUT_GLOBAL;
+    }
+    if (two && s + 1 == np) {
+        return OUT_MID;
+    }
+    return OUT_LDS;
+}
+
+// Run the `count` passes of radix R that come next in the fused program (step s).  After
+// program step 0, `after0()` runs (the prefetch of the next tile).
+template <typename T, int NT, int E, int R, int MODE, class After0>
+__device__ __forceinline__ void run_group(
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, uint32_t count, uint32_t &s,
+    uint32_t total, uint32_t np, int first_in, const cx<T> *pf, After0 after0)
+{
+    constexpr bool two = (MODE & SFH_M_MID) != 0;
+    for (uint32_t k = 0; k < count; ++k, ++s) {
+        const uint32_t pi = (s < np) ? s : s + 1;
+        run_pass<T, NT, E, R, MODE>(a, c, lds, a.pass[pi], s == 0 ? first_in : IN_LDS,
+                                    out_mode_of(s, total, np, two, a.flags), c.wvalid, pf);
+        if (s == 0) {
+            after0();
+        }
+    }
+}
+
+// The whole fused program of one tile: FFT #1 radix groups in kernel order, then (MID
+// modes) FFT #2 groups in reverse order, whose first pass is fused into FFT #1's last.
+template <typename T, int NT, int E, int MODE, class After0>
+__device__ __forceinline__ void run_program(
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, int first_in, const cx<T> *pf,
+    After0 after0)
+{
+    constexpr bool two = (MODE & SFH_M_MID) != 0;
+    const uint32_t np = a.npass;
+    const uint32_t total = two ? 2 * np - 1 : np;
+    const uint8_t *n = a.rcount;
+    uint32_t s = 0;
+    if constexpr (E <= 16) {
+        run_group<T, NT, E, 16, MODE>(a, c, lds, n[0], s, total, np, first_in, pf, after0);
+    }
+    run_group<T, NT, E, 8, MODE>(a, c, lds, n[1], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 4, MODE>(a, c, lds, n[2], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 2, MODE>(a, c, lds, n[3], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 9, MODE>(a, c, lds, n[4], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 3, MODE>(a, c, lds, n[5], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 5, MODE>(a, c, lds, n[6], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 7, MODE>(a, c, lds, n[7], s, total, np, first_in, pf, after0);
+    if constexpr (two) {
+        // FFT #2: reverse order; its first pass (radix of FFT #1's last) already ran fused
+        uint8_t m[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            m[i] = n[i];
+        }
+        for (int i = 7; i >= 0; --i) {
+            if (m[i]) {
+                --m[i];
+                break;
+            }
+        }
+        run_group<T, NT, E, 7, MODE>(a, c, lds, m[7], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 5, MODE>(a, c, lds, m[6], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 3, MODE>(a, c, lds, m[5], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 9, MODE>(a, c, lds, m[4], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 2, MODE>(a, c, lds, m[3], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 4, MODE>(a, c, lds, m[2], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 8, MODE>(a, c, lds, m[1], s, total, np, first_in, pf, after0);
+        if constexpr (E <= 16) {
+            run_group<T, NT, E, 16, MODE>(a, c, lds, m[0], s, total, np, first_in, pf, after0);
+        }
     }
 }
 
@@ -582,57 +803,30 @@ __device__ __forceinline__ void store_via_lds(const sfh_tile_args &a, const tile
             const uint32_t col =
                 (MODE & (SFH_M_T4 | SFH_M_POST)) ? col_of(a, c.w0, c.blo, w) : 0u;
             store_one<T, MODE>(
-                a, c.tdst + static_cast<int32_t>(w) * a.out_w, k, col, lds[w * a.LP + k]);
+                a, c, c.tdst + static_cast<int32_t>(w) * a.out_w, k, col, lds[w * a.LP + k]);
         }
     }
 }
 
-// Waves per SIMD that the LDS footprint allows; used as the register-budget hint.
-template <typename T, int NT, int E>
-struct occupancy
+// Coordinates of tile g and this lane's column in it.
+template <typename T, int MODE>
+__device__ __forceinline__ void set_tile(const sfh_tile_args &a, tile_ctx<T> &c, uint64_t g, uint32_t tid)
 {
-    static constexpr int CAP = NT * E;
-    static constexpr int SLACK = ((CAP + 64) * int(sizeof(cx<T>)) > 163840) ? 0 : 64;
-    static constexpr int BYTES = (CAP + SLACK) * int(sizeof(cx<T>));
-    static constexpr int WG = 163840 / BYTES;
-    static constexpr int RAW = WG * (NT / 64) / 4;
-    static constexpr int WAVES = RAW < 1 ? 1 : (RAW > 8 ? 8 : RAW);
-};
-
-template <typename T, int NT, int E, int MODE>
-__global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(occupancy<T, NT, E>::WAVES)
-void tile_fft(const sfh_tile_args a_byval)
-{
-    // Read the arguments in place in the kernarg segment (constant address space, scalar
-    // loads): indexing a by-value aggregate dynamically would copy it to scratch.
-#if defined(__HIP_DEVICE_COMPILE__)
-    const sfh_tile_args &a = *(const sfh_tile_args *)(__builtin_amdgcn_kernarg_segment_ptr());
-    (void)a_byval;
-#else
-    const sfh_tile_args &a = a_byval;
-#endif
-    __shared__ cx<T> lds[occupancy<T, NT, E>::CAP + occupancy<T, NT, E>::SLACK];
-
-    tile_ctx<T> c;
-    {
-        const uint64_t g = blockIdx.x;
-        const uint64_t wt = g % a.nwt;
-        const uint64_t rest = g / a.nwt;
-        const uint64_t blo = rest % a.nblo;
-        const uint64_t bhi = rest / a.nblo;
-        const uint64_t w0 = wt * a.W;
-        c.w0 = static_cast<uint32_t>(w0);
-        c.blo = static_cast<uint32_t>(blo);
-        const uint64_t left = a.wcount - w0;
-        c.wv = static_cast<uint32_t>(left < a.W ? left : a.W);
-        c.tsrc = static_cast<const cx<T> *>(a.in)
-                 + (static_cast<int64_t>(bhi) * a.in_bhi + static_cast<int64_t>(blo) * a.in_blo
-                    + static_cast<int64_t>(w0) * a.in_w);
-        c.tdst = static_cast<cx<T> *>(a.out)
-                 + (static_cast<int64_t>(bhi) * a.out_bhi + static_cast<int64_t>(blo) * a.out_blo
-                    + static_cast<int64_t>(w0) * a.out_w);
-    }
-    const uint32_t tid = threadIdx.x;
+    const uint64_t wt = g % a.nwt;
+    const uint64_t rest = g / a.nwt;
+    const uint64_t blo = rest % a.nblo;
+    const uint64_t bhi = rest / a.nblo;
+    const uint64_t w0 = wt * a.W;
+    c.w0 = static_cast<uint32_t>(w0);
+    c.blo = static_cast<uint32_t>(blo);
+    const uint64_t left = a.wcount - w0;
+    c.wv = static_cast<uint32_t>(left < a.W ? left : a.W);
+    c.tsrc = static_cast<const cx<T> *>(a.in)
+             + (static_cast<int64_t>(bhi) * a.in_bhi + static_cast<int64_t>(blo) * a.in_blo
+                + static_cast<int64_t>(w0) * a.in_w);
+    c.tdst = static_cast<cx<T> *>(a.out)
+             + (static_cast<int64_t>(bhi) * a.out_bhi + static_cast<int64_t>(blo) * a.out_blo
+                + static_cast<int64_t>(w0) * a.out_w);
     if (a.W == 1) {
         c.w = 0; // NT need not be a power of two here
         c.lt = tid;
@@ -643,35 +837,100 @@ void tile_fft(const sfh_tile_args a_byval)
         c.w = tid >> a.logtpc;
         c.lt = tid & (a.tpc - 1);
     }
-    const bool wvalid = c.w < c.wv;
+    c.wvalid = c.w < c.wv;
     c.src = c.tsrc + static_cast<int32_t>(c.w) * a.in_w;
     c.dst = c.tdst + static_cast<int32_t>(c.w) * a.out_w;
     c.col = (MODE != 0) ? col_of(a, c.w0, c.blo, c.w) : 0u;
+}
+
+// Waves per SIMD that the LDS footprint allows; used as the register-budget hint.
+template <typename T, int NT, int E>
+struct occupancy
+{
+    static constexpr int CAP = NT * E;
+    static constexpr int BYTES = CAP * int(sizeof(cx<T>)) + 1024;
+    static constexpr int WG = 163840 / BYTES;
+    static constexpr int RAW = WG * (NT / 64) / 4;
+    static constexpr int WAVES = RAW < 1 ? 1 : (RAW > 8 ? 8 : RAW);
+};
+
+// PREF = 1: persistent workgroups with register prefetch of the next tile's pass-0 inputs
+// (launched with at most the resident number of workgroups); PREF = 0: pass 0 reads HBM
+// directly (one tile per workgroup, full occupancy).
+template <typename T, int NT, int E, int MODE, int PREF>
+__global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(PREF ? (occupancy<T, NT, E>::WAVES + 1) / 2 : occupancy<T, NT, E>::WAVES)
+void tile_fft(const sfh_tile_args a_byval)
+{
+    // Read the arguments in place in the kernarg segment (constant address space, scalar
+    // loads): indexing a by-value aggregate dynamically would copy it to scratch.
+#if defined(__HIP_DEVICE_COMPILE__)
+    const sfh_tile_args &a = *(const sfh_tile_args *)(__builtin_amdgcn_kernarg_segment_ptr());
+    (void)a_byval;
+#else
+    const sfh_tile_args &a = a_byval;
+#endif
+    SFH_DYNAMIC_LDS(sfh_smem);
+    cx<T> *lds = reinterpret_cast<cx<T> *>(sfh_smem);
+
+    const uint32_t tid = threadIdx.x;
+    tile_ctx<T> c;
+    // stage the twiddle tables (a few KiB) into LDS behind the tile, once per workgroup
+    {
+        cx<T> *tab = lds + a.lds_tab;
+        const cx<T> *src = static_cast<const cx<T> *>(a.twtab);
+        const uint32_t count = a.tw_entries + a.t4_entries;
+        for (uint32_t e = tid; e < count; e += NT) {
+            tab[e] = src[e];
+        }
+        c.tw = tab;
+        c.t4 = tab + a.tw_entries;
+        __syncthreads();
+    }
 
     const uint32_t f = a.flags;
-    int in_mode = IN_GLOBAL;
-    if (f & SFH_LOAD_LDS) {
-        load_via_lds<T, NT, E, MODE>(a, c, lds);
-        in_mode = IN_LDS;
-    }
-    constexpr bool two = (MODE & SFH_M_MID) != 0;
-    const uint32_t np = a.npass;
-    const uint32_t total = two ? 2 * np - 1 : np; // FFT #2 pass 0 is fused into FFT #1's last
-    for (uint32_t s = 0; s < total; ++s) {
-        // fused program: FFT #1 passes 0..np-1, then FFT #2 passes 1..np-1 (stored at np+1..)
-        const uint32_t pi = (s < np) ? s : s + 1;
-        int out_mode;
-        if (s + 1 == total) {
-            out_mode = (f & SFH_STORE_LDS) ? OUT_LDS : OUT_GLOBAL;
-        } else if (two && s + 1 == np) {
-            out_mode = OUT_MID;
-        } else {
-            out_mode = OUT_LDS;
+    if constexpr (!PREF) {
+        // one tile per workgroup; pass 0 reads HBM directly
+        set_tile<T, MODE>(a, c, blockIdx.x, tid);
+        int in_mode = IN_GLOBAL;
+        if (f & SFH_LOAD_LDS) {
+            load_via_lds<T, NT, E, MODE>(a, c, lds);
+            in_mode = IN_LDS;
         }
-        dispatch_pass<T, NT, E, MODE>(a, c, lds, a.pass[pi], s == 0 ? in_mode : IN_LDS, out_mode, wvalid);
-    }
-    if (f & SFH_STORE_LDS) {
-        store_via_lds<T, NT, E, MODE>(a, c, lds);
+        run_program<T, NT, E, MODE>(a, c, lds, in_mode, nullptr, [] {});
+        if (f & SFH_STORE_LDS) {
+            store_via_lds<T, NT, E, MODE>(a, c, lds);
+        }
+    } else {
+        // Persistent loop over tiles g = blockIdx.x, + gridDim.x, ...  The inputs of the next
+        // tile's pass 0 are loaded into registers right after the current tile's pass 0, so
+        // the HBM reads overlap the LDS passes (no pass waits on global memory).
+        uint64_t g = blockIdx.x;
+        if (g >= a.ntiles) {
+            return; // uniform across the workgroup
+        }
+        set_tile<T, MODE>(a, c, g, tid);
+        cx<T> pf[prefetch_size<E>::value];
+        dispatch_prefetch<T, E, MODE>(a, c, pf, c.wvalid);
+        for (;;) {
+            const uint64_t gn = g + gridDim.x;
+            const bool more = gn < a.ntiles;
+            tile_ctx<T> cn = c;
+            run_program<T, NT, E, MODE>(a, c, lds, IN_REGS, pf, [&] {
+                if (more) {
+                    set_tile<T, MODE>(a, cn, gn, tid);
+                    dispatch_prefetch<T, E, MODE>(a, cn, pf, cn.wvalid);
+                }
+            });
+            if (f & SFH_STORE_LDS) {
+                store_via_lds<T, NT, E, MODE>(a, c, lds);
+            }
+            if (!more) {
+                break;
+            }
+            __syncthreads(); // the LDS tile is reused by the next tile
+            g = gn;
+            c = cn;
+        }
     }
 }
 

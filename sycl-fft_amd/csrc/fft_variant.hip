@@ -7,14 +7,17 @@
 #define SFH_CAT2(a, b) a##b
 #define SFH_CAT(a, b) SFH_CAT2(a, b)
 
-// Fusion modes in the order of sfh_mode_index() (fft_kernels.h).
+// Fusion modes in the order of sfh_mode_index (fft_kernels.h): the six one-tile-per-workgroup
+// entries, then the persistent prefetching forms of the plain and four-step modes.
+#define SFH_K(MODE, PREF) reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, MODE, PREF>)
 extern "C" void SFH_CAT(sfh_kernels_, SFH_NAME)(const void **out)
 {
-    out[0] = reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, 0>);
-    out[1] = reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, SFH_M_T4>);
-    out[2] = reinterpret_cast<const void *>(
-        &sfh::tile_fft<SFH_T, SFH_NT, SFH_E, SFH_M_PRE | SFH_M_MID | SFH_M_POST>);
-    out[3] = reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, SFH_M_PRE | SFH_M_T4>);
-    out[4] = reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, SFH_M_MID | SFH_M_T4>);
-    out[5] = reinterpret_cast<const void *>(&sfh::tile_fft<SFH_T, SFH_NT, SFH_E, SFH_M_POST>);
+    out[0] = SFH_K(0, 0);
+    out[1] = SFH_K(SFH_M_T4, 0);
+    out[2] = SFH_K(SFH_M_PRE | SFH_M_MID | SFH_M_POST, 0);
+    out[3] = SFH_K(SFH_M_PRE | SFH_M_T4, 0);
+    out[4] = SFH_K(SFH_M_MID | SFH_M_T4, 0);
+    out[5] = SFH_K(SFH_M_POST, 0);
+    out[6] = SFH_K(0, 1);
+    out[7] = SFH_K(SFH_M_T4, 1);
 }

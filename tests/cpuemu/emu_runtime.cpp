@@ -26,6 +26,16 @@ hipError_t g_last = hipSuccess;
 int g_device = 0;
 } // namespace
 
+namespace
+{
+unsigned char *g_dyn_lds = nullptr;
+}
+
+unsigned char *emu_dynamic_lds()
+{
+    return g_dyn_lds;
+}
+
 void emu_syncthreads()
 {
     g_bar->arrive_and_wait();
@@ -56,15 +66,20 @@ void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body)
     g_bar = nullptr;
 }
 
-hipError_t hipLaunchKernel(const void *fn, dim3 grid, dim3 block, void **args, size_t, hipStream_t)
+hipError_t hipLaunchKernel(const void *fn, dim3 grid, dim3 block, void **args, size_t shmem, hipStream_t)
 {
     // every kernel launched through this entry takes one sfh_tile_args by value
     auto kernel = reinterpret_cast<void (*)(sfh_tile_args)>(const_cast<void *>(fn));
     const sfh_tile_args a = *static_cast<const sfh_tile_args *>(args[0]);
-    if (block.x == 0 || block.x > 1024 || grid.x == 0) {
+    if (block.x == 0 || block.x > 1024 || grid.x == 0 || shmem > 163840) {
         return hipErrorInvalidConfiguration;
     }
+    // exactly the requested dynamic LDS, so AddressSanitizer flags any access past it
+    unsigned char *lds = static_cast<unsigned char *>(std::malloc(shmem ? shmem : 1));
+    g_dyn_lds = lds;
     emu_launch(grid, block, [&]() { kernel(a); });
+    g_dyn_lds = nullptr;
+    std::free(lds);
     return hipSuccess;
 }
 
@@ -200,4 +215,34 @@ hipError_t hipGetLastError()
 const char *hipGetErrorString(hipError_t e)
 {
     return e == hipSuccess ? "success" : "emulated HIP error";
+}
+
+hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int)
+{
+    return hipSuccess;
+}
+hipError_t hipEventQuery(hipEvent_t)
+{
+    return hipSuccess;
+}
+hipError_t hipStreamCreate(hipStream_t *s)
+{
+    *s = nullptr;
+    return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t)
+{
+    return hipSuccess;
+}
+
+// a small emulated device: 2 CUs, 1 resident workgroup each, so persistent launches loop
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int)
+{
+    *v = 2;
+    return hipSuccess;
+}
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t)
+{
+    *n = 1;
+    return hipSuccess;
 }

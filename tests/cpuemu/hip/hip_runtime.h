@@ -19,6 +19,8 @@ extern thread_local dim3 blockDim;
 extern thread_local dim3 gridDim;
 
 void emu_syncthreads();
+// dynamic LDS of the running launch (sized by the launch's sharedMem argument)
+unsigned char *emu_dynamic_lds();
 #define __syncthreads() emu_syncthreads()
 
 inline unsigned __umulhi(unsigned a, unsigned b)

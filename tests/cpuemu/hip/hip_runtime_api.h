@@ -48,6 +48,10 @@ struct hipDeviceProp_t
     int multiProcessorCount;
 };
 #define hipEventDisableTiming 2u
+enum hipFuncAttribute
+{
+    hipFuncAttributeMaxDynamicSharedMemorySize = 8
+};
 
 struct dim3
 {
@@ -78,3 +82,13 @@ hipError_t hipMemGetAddressRange(hipDeviceptr_t *, size_t *, hipDeviceptr_t);
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t);
 hipError_t hipLaunchKernel(const void *, dim3, dim3, void **, size_t, hipStream_t);
+hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int);
+enum hipDeviceAttribute_t
+{
+    hipDeviceAttributeMultiprocessorCount = 63
+};
+hipError_t hipDeviceGetAttribute(int *, hipDeviceAttribute_t, int);
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *, const void *, int, size_t);
+hipError_t hipEventQuery(hipEvent_t);
+hipError_t hipStreamCreate(hipStream_t *);
+hipError_t hipStreamDestroy(hipStream_t);

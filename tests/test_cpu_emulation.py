@@ -49,3 +49,14 @@ def test_emulated_launch_kinds():
 
 def test_emulated_multi_launch_paths():
     _run(CAPPED, {"SYCLFFT_HIP_MAX_TILE": "1024"})
+
+
+CHUNKED = ["f:65536:9:-1:0:0", "d:32768:5:1:3:1", "f:5x65536:1:1:1:0", "f:65536x3:2:-1:0:0"]
+CHUNKED_CAPPED = ["f:4099:40:1:3:0", "d:1031x3:40:1:1:1"]
+
+
+def test_emulated_chunked_launch_groups():
+    """Multi-launch axes split into launch groups of ~1 MiB (SYCLFFT_HIP_CHUNK_MB): the
+    direction conjugations and the scale must reach every group."""
+    _run(CHUNKED, {"SYCLFFT_HIP_CHUNK_MB": "1"})
+    _run(CHUNKED_CAPPED, {"SYCLFFT_HIP_CHUNK_MB": "1", "SYCLFFT_HIP_MAX_TILE": "1024"})
