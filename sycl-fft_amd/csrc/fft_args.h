@@ -90,9 +90,11 @@ struct sfh_tile_args
     uint32_t tpc;    // threads per column = NT / W
     uint32_t logtpc;
     uint32_t npass;  // passes of FFT #1 (FFT #2 uses the reversed radix order)
-    // passes per radix in the fixed kernel order 16, 8, 4, 2, 9, 3, 5, 7 (FFT #1 runs the
-    // radices in this order, FFT #2 in the reverse order)
+    // passes per radix in the fixed kernel order 16, 8, 4, 2, 9, 3, 5, 7, without pass 0
+    // (FFT #1 runs them in this order after pass 0); rcount2: FFT #2's passes without its
+    // first one, which is fused into FFT #1's last (FFT #2 runs them in the reverse order)
     uint8_t rcount[8];
+    uint8_t rcount2[8];
     uint32_t flags;
     sfh_fastdiv L_div;
     sfh_fastdiv col_div, col_mod;

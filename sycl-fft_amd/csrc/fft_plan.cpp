@@ -14,6 +14,7 @@
 // direction are fused into the first/last launch; Bluestein uses the smallest 7-smooth
 // convolution length >= 2n-1 instead of the next power of two.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <cmath>
@@ -207,9 +208,40 @@ struct launch
     std::string desc;
 };
 
+// Planner choices the autotuner searches over (planning_mode::measure).
+struct tuning
+{
+    int split{0};    // 1: multi-launch form (four-step, Bluestein A/B/C) even where one launch fits
+    int wshift{0};   // column-tile width scaled by 2^wshift (-1, 0, +1)
+    int prefetch{0}; // persistent prefetching form of plain / four-step-A launches
+    int grow{0};     // the next larger kernel variant (more work-items per tile)
+
+    std::string str() const
+    {
+        return "s" + std::to_string(split) + "w" + std::to_string(wshift) + "p" + std::to_string(prefetch) + "g"
+               + std::to_string(grow);
+    }
+    static bool parse(const std::string &text, tuning &t)
+    {
+        int s = 0, w = 0, pf = 0, g = 0;
+        if (std::sscanf(text.c_str(), "s%dw%dp%dg%d", &s, &w, &pf, &g) != 4) {
+            return false;
+        }
+        if (s < 0 || s > 1 || w < -1 || w > 1 || pf < 0 || pf > 1 || g < 0 || g > 1) {
+            return false;
+        }
+        t.split = s;
+        t.wshift = w;
+        t.prefetch = pf;
+        t.grow = g;
+        return true;
+    }
+};
+
 struct plan
 {
     syclfft_hip_plan_config_v1 cfg{};
+    tuning tune{};
     int device{};
     bool dbl{};
     size_t esize{};
@@ -316,6 +348,25 @@ static int pick_variant(const plan &p, uint32_t L, uint32_t W, const tw_layout &
             best = i;
         }
     }
+    if (best >= 0 && p.tune.grow) {
+        // the next larger qualifying variant, if any
+        int next = -1;
+        const int bcap = p.variants[best].nt * p.variants[best].e;
+        for (int i = 0; i < p.nvariants; ++i) {
+            const int cap = p.variants[i].nt * p.variants[i].e;
+            if (!variant_allowed(p, i) || cap <= bcap || lds_bytes_of(p, L, W, t, nullptr) > k_lds_limit
+                || static_cast<uint32_t>(p.variants[i].nt) < W
+                || (W > 1 && !is_pow2(static_cast<uint64_t>(p.variants[i].nt)))) {
+                continue;
+            }
+            if (next < 0 || cap < p.variants[next].nt * p.variants[next].e) {
+                next = i;
+            }
+        }
+        if (next >= 0) {
+            best = next;
+        }
+    }
     return best;
 }
 
@@ -400,16 +451,33 @@ static void set_passes(launch &l, uint32_t L, bool second_fft, uint32_t maxr)
     // per-radix pass counts in the kernel's fixed order; radix_plan emits that order
     static const uint32_t order[8] = {16, 8, 4, 2, 9, 3, 5, 7};
     uint32_t pos = 0;
+    uint32_t counts[8] = {};
     for (int k = 0; k < 8; ++k) {
-        uint32_t cnt = 0;
         while (pos < radices.size() && radices[pos] == order[k]) {
-            ++cnt;
+            ++counts[k];
             ++pos;
         }
-        l.args.rcount[k] = static_cast<uint8_t>(cnt);
     }
     if (pos != radices.size()) {
         throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "radix program out of kernel order");
+    }
+    // FFT #1 without pass 0 (the first pass of the first non-empty group); FFT #2 without its
+    // first pass (the last radix of FFT #1, i.e. the last non-empty group)
+    for (int k = 0; k < 8; ++k) {
+        l.args.rcount[k] = static_cast<uint8_t>(counts[k]);
+        l.args.rcount2[k] = static_cast<uint8_t>(counts[k]);
+    }
+    for (int k = 0; k < 8; ++k) {
+        if (counts[k]) {
+            --l.args.rcount[k];
+            break;
+        }
+    }
+    for (int k = 7; k >= 0; --k) {
+        if (counts[k]) {
+            --l.args.rcount2[k];
+            break;
+        }
     }
     if (second_fft && np < 2) {
         throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "Bluestein length needs two radix passes");
@@ -530,6 +598,11 @@ static uint32_t choose_w(const plan &p, uint64_t L, uint64_t wcount, bool contig
             w *= 2;
         }
     }
+    if (p.tune.wshift > 0 && w < 64) {
+        w *= 2;
+    } else if (p.tune.wshift < 0 && w > 1) {
+        w /= 2;
+    }
     while (w > 1 && w * (L + 1) > cap) {
         w /= 2;
     }
@@ -616,7 +689,9 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
                 g.in_j = g.out_k = static_cast<int64_t>(inner);
                 g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
             }
-            if (n <= static_cast<uint64_t>(max_cap(p)) && fits_single(p, n, g.W)) {
+            uint64_t s1 = 0, s2 = 0;
+            const bool can_split = p.tune.split && split_two(p, n, 1, s1, s2);
+            if (!can_split && n <= static_cast<uint64_t>(max_cap(p)) && fits_single(p, n, g.W)) {
                 p.launches.push_back(make_launch(p, SFH_MI_PLAIN, g, src, dst, false, "single"));
                 return;
             }
@@ -701,7 +776,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
     // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
     const uint64_t need = 2 * n - 1;
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
-    uint64_t M = smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+    uint64_t M = p.tune.split ? 0 : smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
     const bool fused = M != 0;
     uint64_t m1 = 0, m2 = 0;
     if (!fused) {
@@ -1012,15 +1087,10 @@ static void release(plan *p)
     delete p;
 }
 
-static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
+static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune)
 {
-    validate_config(cfg);
-    std::string reason;
-    const int dev = check_device(cfg->device, reason);
-    if (dev < 0) {
-        throw sfh_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, reason);
-    }
     std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
+    p->tune = tune;
     p->cfg = *cfg;
     p->device = dev;
     p->dbl = cfg->scalar == SYCLFFT_HIP_SCALAR_DOUBLE;
@@ -1091,7 +1161,7 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
     // per workgroup at full occupancy on the batched shapes, so off by default).
     {
         const char *pe = std::getenv("SYCLFFT_HIP_PREFETCH");
-        const bool allow = pe && std::atoi(pe) != 0;
+        const bool allow = pe ? std::atoi(pe) != 0 : p->tune.prefetch != 0;
         const int cus = sfh_device_cus();
         for (auto &l : p->launches) {
             if (!allow || cus <= 0 || (l.args.flags & SFH_LOAD_LDS)) {
@@ -1135,6 +1205,252 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
         trace(l.desc);
     }
     return p.release();
+}
+
+// ------------------------------------------------------------------------------------
+// Autotuner: planning_mode::measure times candidate tunings on the device and caches the
+// fastest per (device, shape) in a text file ($SYCLFFT_HIP_TUNE_CACHE, default
+// ~/.cache/syclfft_hip/tuning_v1.txt, lines "<key>\t<tuning>").  Plans whose shape is in
+// the cache use the cached tuning in every planning mode.  SYCLFFT_HIP_TUNE=<tuning> forces
+// one (tests, experiments).
+// ------------------------------------------------------------------------------------
+
+static std::string tune_cache_path()
+{
+    if (const char *e = std::getenv("SYCLFFT_HIP_TUNE_CACHE")) {
+        return e;
+    }
+    const char *home = std::getenv("HOME");
+    return std::string(home && home[0] ? home : "/tmp") + "/.cache/syclfft_hip/tuning_v1.txt";
+}
+
+static std::mutex &tune_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+static std::map<std::string, std::string> &tune_cache()
+{
+    static std::map<std::string, std::string> cache;
+    static bool loaded = false;
+    if (!loaded) {
+        loaded = true;
+        if (FILE *f = std::fopen(tune_cache_path().c_str(), "r")) {
+            char line[512];
+            while (std::fgets(line, sizeof(line), f)) {
+                std::string l(line);
+                while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) {
+                    l.pop_back();
+                }
+                const auto tab = l.find('\t');
+                if (tab != std::string::npos) {
+                    cache[l.substr(0, tab)] = l.substr(tab + 1);
+                }
+            }
+            std::fclose(f);
+        }
+    }
+    return cache;
+}
+
+static void tune_store(const std::string &key, const std::string &value)
+{
+    tune_cache()[key] = value;
+    const std::string path = tune_cache_path();
+    const auto slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        // create the directory chain (best effort)
+        std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); ++i) {
+            if (i == dir.size() || dir[i] == '/') {
+                (void)::mkdir(dir.substr(0, i).c_str(), 0755);
+            }
+        }
+    }
+    if (FILE *f = std::fopen(path.c_str(), "a")) {
+        std::fprintf(f, "%s\t%s\n", key.c_str(), value.c_str());
+        std::fclose(f);
+    }
+}
+
+static std::string tune_key(const syclfft_hip_plan_config_v1 *cfg, int dev)
+{
+    hipDeviceProp_t prop{};
+    (void)hipGetDeviceProperties(&prop, dev);
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%.6s:%d|%s|%d|%u:%llux%llux%llu|%llu|%u|%u|%u", prop.gcnArchName,
+                  prop.multiProcessorCount, cfg->scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? "f64" : "f32",
+                  cfg->direction, cfg->rank, static_cast<unsigned long long>(cfg->lengths[0]),
+                  static_cast<unsigned long long>(cfg->rank > 1 ? cfg->lengths[1] : 1),
+                  static_cast<unsigned long long>(cfg->rank > 2 ? cfg->lengths[2] : 1),
+                  static_cast<unsigned long long>(cfg->batch_count), cfg->in_place, cfg->normalization,
+                  cfg->precision);
+    return buf;
+}
+
+static void execute_plan(plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
+                         hipEvent_t done);
+
+// Milliseconds per execution of a trial plan, or a negative value if it cannot be timed.
+static double time_plan(plan &p)
+{
+    const uint64_t bytes = p.total * p.esize;
+    void *in = nullptr, *out = nullptr;
+    hipStream_t st{};
+    hipEvent_t e0{}, e1{};
+    double ms = -1.0;
+    if (hipMalloc(&in, bytes) != hipSuccess || (!p.cfg.in_place && hipMalloc(&out, bytes) != hipSuccess)) {
+        (void)hipGetLastError();
+    } else if (hipMemset(in, 0, bytes) == hipSuccess && hipStreamCreate(&st) == hipSuccess
+               && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        void *o = p.cfg.in_place ? in : out;
+        try {
+            for (int i = 0; i < 2; ++i) {
+                execute_plan(p, in, o, st, nullptr, 0, nullptr);
+            }
+            const int reps = 5;
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < reps; ++i) {
+                execute_plan(p, in, o, st, nullptr, 0, nullptr);
+            }
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) == hipSuccess) {
+                float t = 0;
+                if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
+                    ms = t / reps;
+                }
+            }
+        } catch (const std::exception &) {
+            ms = -1.0;
+        }
+        (void)hipStreamSynchronize(st);
+    }
+    if (p.have_last) {
+        (void)hipEventSynchronize(p.last);
+    }
+    if (e0) {
+        (void)hipEventDestroy(e0);
+    }
+    if (e1) {
+        (void)hipEventDestroy(e1);
+    }
+    if (st) {
+        (void)hipStreamDestroy(st);
+    }
+    if (in) {
+        (void)hipFree(in);
+    }
+    if (out) {
+        (void)hipFree(out);
+    }
+    return ms;
+}
+
+static std::string describe_plan(const plan &p);
+
+// Greedy search: the default, each single change, then the pairs of improving changes.
+static tuning autotune(const syclfft_hip_plan_config_v1 *cfg, int dev)
+{
+    std::map<std::string, double> seen; // launch sequence -> ms
+    auto trial = [&](const tuning &t) -> double {
+        std::unique_ptr<plan, void (*)(plan *)> q(build_plan(cfg, dev, t), release);
+        const std::string d = describe_plan(*q);
+        const auto it = seen.find(d);
+        if (it != seen.end()) {
+            return it->second;
+        }
+        const double ms = time_plan(*q);
+        seen[d] = ms;
+        trace("tune " + t.str() + " " + std::to_string(ms) + " ms");
+        return ms;
+    };
+    tuning best{};
+    double best_ms = trial(best);
+    std::vector<tuning> singles;
+    for (int k = 0; k < 5; ++k) {
+        tuning t{};
+        if (k == 0) {
+            t.split = 1;
+        } else if (k == 1) {
+            t.wshift = -1;
+        } else if (k == 2) {
+            t.wshift = 1;
+        } else if (k == 3) {
+            t.prefetch = 1;
+        } else {
+            t.grow = 1;
+        }
+        singles.push_back(t);
+    }
+    std::vector<tuning> better;
+    for (const auto &t : singles) {
+        double ms = -1;
+        try {
+            ms = trial(t);
+        } catch (const std::exception &) {
+            continue;
+        }
+        if (ms > 0 && (best_ms <= 0 || ms < best_ms)) {
+            better.push_back(t);
+        }
+        if (ms > 0 && (best_ms <= 0 || ms < best_ms * 0.98)) {
+            best = t;
+            best_ms = ms;
+        }
+    }
+    for (size_t i = 0; i < better.size(); ++i) {
+        for (size_t j = i + 1; j < better.size(); ++j) {
+            tuning t = better[i];
+            const tuning &u = better[j];
+            t.split |= u.split;
+            t.prefetch |= u.prefetch;
+            t.grow |= u.grow;
+            if (u.wshift) {
+                t.wshift = u.wshift;
+            }
+            double ms = -1;
+            try {
+                ms = trial(t);
+            } catch (const std::exception &) {
+                continue;
+            }
+            if (ms > 0 && ms < best_ms * 0.98) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+    }
+    return best;
+}
+
+static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
+{
+    validate_config(cfg);
+    std::string reason;
+    const int dev = check_device(cfg->device, reason);
+    if (dev < 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, reason);
+    }
+    tuning t{};
+    if (const char *forced = std::getenv("SYCLFFT_HIP_TUNE")) {
+        if (!tuning::parse(forced, t)) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "SYCLFFT_HIP_TUNE is not s<0|1>w<-1|0|1>p<0|1>g<0|1>");
+        }
+    } else {
+        device_guard guard(dev);
+        const std::string key = tune_key(cfg, dev);
+        std::lock_guard<std::mutex> lock(tune_mutex());
+        const auto &cache = tune_cache();
+        const auto it = cache.find(key);
+        if (it != cache.end() && tuning::parse(it->second, t)) {
+            // cached
+        } else if (cfg->measure) {
+            t = autotune(cfg, dev);
+            tune_store(key, t.str());
+        }
+    }
+    return build_plan(cfg, dev, t);
 }
 
 // Device-pointer check: the whole [ptr, ptr+bytes) range must lie in one device
@@ -1227,6 +1543,20 @@ static void execute_plan(
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
     }
+}
+
+static std::string describe_plan(const plan &p)
+{
+    std::string s;
+    if (p.copy_only) {
+        s += "copy_scale\n";
+    }
+    for (const auto &l : p.launches) {
+        s += l.desc;
+        s += '\n';
+    }
+    s += "tuning " + p.tune.str() + "\n";
+    return s;
 }
 
 } // namespace sfh
@@ -1346,15 +1676,7 @@ extern "C" size_t syclfft_hip_plan_describe(const void *plan, char *buf, size_t 
     if (!plan) {
         return 0;
     }
-    const auto *p = static_cast<const sfh::plan *>(plan);
-    std::string s;
-    if (p->copy_only) {
-        s += "copy_scale\n";
-    }
-    for (const auto &l : p->launches) {
-        s += l.desc;
-        s += '\n';
-    }
+    const std::string s = sfh::describe_plan(*static_cast<const sfh::plan *>(plan));
     if (buf && cap) {
         const size_t n = std::min(cap - 1, s.size());
         std::memcpy(buf, s.data(), n);

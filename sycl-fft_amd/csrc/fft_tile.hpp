@@ -495,15 +495,36 @@ __device__ __forceinline__ void prefetch_pass0(
     }
 }
 
-template <typename T, int NT, int E, int R, int MODE>
+struct no_hook
+{
+    __device__ void operator()() const {}
+};
+
+// after_loads() runs once every lane has issued its pass-0 HBM loads (IN_GLOBAL), before
+// they are consumed: the twiddle tables are staged there so both round trips overlap.
+template <typename T, int NT, int E, int R, int MODE, class Hook = no_hook>
 __device__ __forceinline__ void run_pass(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
-    int out_mode, bool wvalid, const cx<T> *pf)
+    int out_mode, bool wvalid, const cx<T> *pf, Hook after_loads = Hook())
 {
     constexpr int NB = (E + R - 1) / R;
     const uint32_t m = a.L / R; // butterflies per column
     cx<T> *ldsc = lds + c.w * a.LP;
     cx<T> v[NB][R];
+
+    if (in_mode == IN_GLOBAL) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+            if (wvalid && i < m) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    v[q][t] = load_one<T, MODE>(a, c.src, i + t * m, c.col);
+                }
+            }
+        }
+        after_loads();
+    }
 
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -515,10 +536,7 @@ __device__ __forceinline__ void run_pass(
                     v[q][t] = pf[q * R + t];
                 }
             } else if (in_mode == IN_GLOBAL) {
-#pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    v[q][t] = load_one<T, MODE>(a, c.src, i + t * m, c.col);
-                }
+                // loaded above
             } else {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
@@ -564,6 +582,7 @@ __device__ __forceinline__ void run_pass(
                     }
                 }
             }
+            SFH_SCHED_BARRIER();
         }
         return;
     }
@@ -582,6 +601,7 @@ __device__ __forceinline__ void run_pass(
                     }
                     dft<T, R>::run(v[q]);
                 }
+                SFH_SCHED_BARRIER();
             }
         }
     }
@@ -643,38 +663,38 @@ __device__ __forceinline__ void dispatch_prefetch(
     }
 }
 
-template <typename T, int NT, int E, int MODE>
+template <typename T, int NT, int E, int MODE, class Hook = no_hook>
 __device__ __forceinline__ void dispatch_pass(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
-    int out_mode, bool wvalid, const cx<T> *pf)
+    int out_mode, bool wvalid, const cx<T> *pf, Hook hook = Hook())
 {
     switch (p.radix) {
     case 2:
-        run_pass<T, NT, E, 2, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 2, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 3:
-        run_pass<T, NT, E, 3, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 3, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 4:
-        run_pass<T, NT, E, 4, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 4, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 5:
-        run_pass<T, NT, E, 5, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 5, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 7:
-        run_pass<T, NT, E, 7, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 7, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 8:
-        run_pass<T, NT, E, 8, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 8, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     case 9:
-        run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+        run_pass<T, NT, E, 9, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         break;
     default:
         // radix 16 only where at most 16 elements per lane are staged (register budget);
         // the planner never gives larger-E variants a radix above 9
         if constexpr (E <= 16) {
-            run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf);
+            run_pass<T, NT, E, 16, MODE>(a, c, lds, p, in_mode, out_mode, wvalid, pf, hook);
         }
         break;
     }
@@ -692,68 +712,59 @@ __device__ __forceinline__ int out_mode_of(uint32_t s, uint32_t total, uint32_t 
     return OUT_LDS;
 }
 
-// Run the `count` passes of radix R that come next in the fused program (step s).  After
-// program step 0, `after0()` runs (the prefetch of the next tile).
-template <typename T, int NT, int E, int R, int MODE, class After0>
+// Run the `count` passes of radix R (inputs in LDS) that come next in the fused program.
+template <typename T, int NT, int E, int R, int MODE>
 __device__ __forceinline__ void run_group(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, uint32_t count, uint32_t &s,
-    uint32_t total, uint32_t np, int first_in, const cx<T> *pf, After0 after0)
+    uint32_t total, uint32_t np)
 {
     constexpr bool two = (MODE & SFH_M_MID) != 0;
     for (uint32_t k = 0; k < count; ++k, ++s) {
         const uint32_t pi = (s < np) ? s : s + 1;
-        run_pass<T, NT, E, R, MODE>(a, c, lds, a.pass[pi], s == 0 ? first_in : IN_LDS,
-                                    out_mode_of(s, total, np, two, a.flags), c.wvalid, pf);
-        if (s == 0) {
-            after0();
-        }
+        run_pass<T, NT, E, R, MODE>(a, c, lds, a.pass[pi], IN_LDS, out_mode_of(s, total, np, two, a.flags),
+                                    c.wvalid, nullptr);
     }
 }
 
 // The whole fused program of one tile: FFT #1 radix groups in kernel order, then (MID
 // modes) FFT #2 groups in reverse order, whose first pass is fused into FFT #1's last.
-template <typename T, int NT, int E, int MODE, class After0>
+template <typename T, int NT, int E, int MODE, class After0, class Hook = no_hook>
 __device__ __forceinline__ void run_program(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, int first_in, const cx<T> *pf,
-    After0 after0)
+    After0 after0, Hook hook = Hook())
 {
     constexpr bool two = (MODE & SFH_M_MID) != 0;
     const uint32_t np = a.npass;
     const uint32_t total = two ? 2 * np - 1 : np;
+    // pass 0 on its own (its inputs come from registers or HBM), so the pass-0 register
+    // buffer is dead before the radix groups run
+    dispatch_pass<T, NT, E, MODE>(a, c, lds, a.pass[0], first_in, out_mode_of(0, total, np, two, a.flags),
+                                  c.wvalid, pf, hook);
+    after0();
     const uint8_t *n = a.rcount;
-    uint32_t s = 0;
+    uint32_t s = 1;
     if constexpr (E <= 16) {
-        run_group<T, NT, E, 16, MODE>(a, c, lds, n[0], s, total, np, first_in, pf, after0);
+        run_group<T, NT, E, 16, MODE>(a, c, lds, n[0], s, total, np);
     }
-    run_group<T, NT, E, 8, MODE>(a, c, lds, n[1], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 4, MODE>(a, c, lds, n[2], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 2, MODE>(a, c, lds, n[3], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 9, MODE>(a, c, lds, n[4], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 3, MODE>(a, c, lds, n[5], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 5, MODE>(a, c, lds, n[6], s, total, np, first_in, pf, after0);
-    run_group<T, NT, E, 7, MODE>(a, c, lds, n[7], s, total, np, first_in, pf, after0);
+    run_group<T, NT, E, 8, MODE>(a, c, lds, n[1], s, total, np);
+    run_group<T, NT, E, 4, MODE>(a, c, lds, n[2], s, total, np);
+    run_group<T, NT, E, 2, MODE>(a, c, lds, n[3], s, total, np);
+    run_group<T, NT, E, 9, MODE>(a, c, lds, n[4], s, total, np);
+    run_group<T, NT, E, 3, MODE>(a, c, lds, n[5], s, total, np);
+    run_group<T, NT, E, 5, MODE>(a, c, lds, n[6], s, total, np);
+    run_group<T, NT, E, 7, MODE>(a, c, lds, n[7], s, total, np);
     if constexpr (two) {
         // FFT #2: reverse order; its first pass (radix of FFT #1's last) already ran fused
-        uint8_t m[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            m[i] = n[i];
-        }
-        for (int i = 7; i >= 0; --i) {
-            if (m[i]) {
-                --m[i];
-                break;
-            }
-        }
-        run_group<T, NT, E, 7, MODE>(a, c, lds, m[7], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 5, MODE>(a, c, lds, m[6], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 3, MODE>(a, c, lds, m[5], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 9, MODE>(a, c, lds, m[4], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 2, MODE>(a, c, lds, m[3], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 4, MODE>(a, c, lds, m[2], s, total, np, first_in, pf, after0);
-        run_group<T, NT, E, 8, MODE>(a, c, lds, m[1], s, total, np, first_in, pf, after0);
+        const uint8_t *m = a.rcount2;
+        run_group<T, NT, E, 7, MODE>(a, c, lds, m[7], s, total, np);
+        run_group<T, NT, E, 5, MODE>(a, c, lds, m[6], s, total, np);
+        run_group<T, NT, E, 3, MODE>(a, c, lds, m[5], s, total, np);
+        run_group<T, NT, E, 9, MODE>(a, c, lds, m[4], s, total, np);
+        run_group<T, NT, E, 2, MODE>(a, c, lds, m[3], s, total, np);
+        run_group<T, NT, E, 4, MODE>(a, c, lds, m[2], s, total, np);
+        run_group<T, NT, E, 8, MODE>(a, c, lds, m[1], s, total, np);
         if constexpr (E <= 16) {
-            run_group<T, NT, E, 16, MODE>(a, c, lds, m[0], s, total, np, first_in, pf, after0);
+            run_group<T, NT, E, 16, MODE>(a, c, lds, m[0], s, total, np);
         }
     }
 }
@@ -874,29 +885,33 @@ void tile_fft(const sfh_tile_args a_byval)
 
     const uint32_t tid = threadIdx.x;
     tile_ctx<T> c;
-    // stage the twiddle tables (a few KiB) into LDS behind the tile, once per workgroup
-    {
-        cx<T> *tab = lds + a.lds_tab;
-        const cx<T> *src = static_cast<const cx<T> *>(a.twtab);
-        const uint32_t count = a.tw_entries + a.t4_entries;
-        for (uint32_t e = tid; e < count; e += NT) {
-            tab[e] = src[e];
-        }
-        c.tw = tab;
-        c.t4 = tab + a.tw_entries;
-        __syncthreads();
-    }
+    cx<T> *const tab = lds + a.lds_tab;
+    const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
+    const uint32_t tab_count = a.tw_entries + a.t4_entries;
+    c.tw = tab;
+    c.t4 = tab + a.tw_entries;
 
     const uint32_t f = a.flags;
-    if constexpr (!PREF) {
-        // one tile per workgroup; pass 0 reads HBM directly
-        set_tile<T, MODE>(a, c, blockIdx.x, tid);
-        int in_mode = IN_GLOBAL;
-        if (f & SFH_LOAD_LDS) {
-            load_via_lds<T, NT, E, MODE>(a, c, lds);
-            in_mode = IN_LDS;
+    // Twiddle tables (a few KiB) go to LDS behind the tile.  They are loaded right after the
+    // tile's pass-0 loads have been issued, so the two HBM/L2 round trips overlap instead of
+    // running one after the other.
+    const auto finish_tables = [=]() {
+        for (uint32_t e = tid; e < tab_count; e += NT) {
+            tab[e] = tab_src[e];
         }
-        run_program<T, NT, E, MODE>(a, c, lds, in_mode, nullptr, [] {});
+        __syncthreads();
+    };
+
+    if constexpr (!PREF) {
+        // one tile per workgroup
+        set_tile<T, MODE>(a, c, blockIdx.x, tid);
+        if (f & SFH_LOAD_LDS) {
+            finish_tables();
+            load_via_lds<T, NT, E, MODE>(a, c, lds);
+            run_program<T, NT, E, MODE>(a, c, lds, IN_LDS, nullptr, [] {});
+        } else {
+            run_program<T, NT, E, MODE>(a, c, lds, IN_GLOBAL, nullptr, [] {}, finish_tables);
+        }
         if (f & SFH_STORE_LDS) {
             store_via_lds<T, NT, E, MODE>(a, c, lds);
         }
@@ -911,6 +926,7 @@ void tile_fft(const sfh_tile_args a_byval)
         set_tile<T, MODE>(a, c, g, tid);
         cx<T> pf[prefetch_size<E>::value];
         dispatch_prefetch<T, E, MODE>(a, c, pf, c.wvalid);
+        finish_tables();
         for (;;) {
             const uint64_t gn = g + gridDim.x;
             const bool more = gn < a.ntiles;

@@ -46,6 +46,7 @@ static bool run_case(const std::string &spec)
     cfg.in_place = static_cast<uint32_t>(inplace);
     cfg.normalization = static_cast<uint32_t>(norm);
     cfg.device = -1;
+    cfg.measure = std::getenv("EMU_MEASURE") ? 1u : 0u;
     char err[1024];
     void *plan = nullptr;
     const int st = syclfft_hip_plan_create_ex(&cfg, &plan, err, sizeof(err));

@@ -71,6 +71,9 @@ hipError_t hipMalloc(void **, size_t);
 hipError_t hipFree(void *);
 hipError_t hipMemcpy(void *, const void *, size_t, hipMemcpyKind);
 hipError_t hipMemsetAsync(void *, int, size_t, hipStream_t);
+hipError_t hipMemset(void *, int, size_t);
+hipError_t hipEventCreate(hipEvent_t *);
+hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 hipError_t hipEventCreateWithFlags(hipEvent_t *, unsigned);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipEventRecord(hipEvent_t, hipStream_t);

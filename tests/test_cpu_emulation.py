@@ -60,3 +60,21 @@ def test_emulated_chunked_launch_groups():
     direction conjugations and the scale must reach every group."""
     _run(CHUNKED, {"SYCLFFT_HIP_CHUNK_MB": "1"})
     _run(CHUNKED_CAPPED, {"SYCLFFT_HIP_CHUNK_MB": "1", "SYCLFFT_HIP_MAX_TILE": "1024"})
+
+
+TUNED = ["f:4096:9:-1:3:0", "d:2520:5:1:3:1", "f:10007:3:1:1:0", "f:32x64x16:2:-1:0:1", "d:16384:2:1:2:0"]
+
+
+@pytest.mark.parametrize("tune", ["s1w0p0g0", "s0w-1p0g0", "s0w1p0g0", "s0w0p1g0", "s0w0p0g1", "s1w1p1g1"])
+def test_emulated_tuning_choices(tune):
+    """Every planner choice the autotuner can make gives a correct, in-bounds plan."""
+    _run(TUNED, {"SYCLFFT_HIP_TUNE": tune})
+
+
+def test_emulated_measure_mode_writes_cache(tmp_path):
+    cache = tmp_path / "tune.txt"
+    _run(["f:4096:9:-1:3:0"], {"EMU_MEASURE": "1", "SYCLFFT_HIP_TUNE_CACHE": str(cache)})
+    text = cache.read_text()
+    assert "f32" in text and "4096x1x1|9" in text
+    key, value = text.strip().split("\t")
+    assert value.startswith("s") and "w" in value
