@@ -5,7 +5,10 @@ forward + backward with orthogonal normalization of
     complex<double> N=2520 (2^3 3^2 5 7) batch 1024,
     complex<double> N=6561 (3^8)         batch 512,
     complex<float>  N=10007 (prime, Bluestein) batch 128,
-six plan executions on device-resident inputs (uniform[-1,1) re/im, seeded).
+six plan executions on device-resident inputs (uniform[-1,1) re/im, seeded).  Plans are
+created with planning_mode::measure (autotuned at set-up, outside the timed region).  The
+timed step runs the six executions back to back on one stream; the same step with the
+three independent sub-workloads on three overlapping streams is reported beside it.
 Metric: C2C GFLOP/s = sum 5 N log2(N) B / time, plus % of HBM peak from algorithmic
 bytes (one read + one write of every transform per execution).
 
@@ -15,7 +18,7 @@ fraction per N), rel-L2 of each sub-workload against the FP64 oracle on sampled 
 the reference-algorithm CPU baseline (the oracle's restatement of the portable provider,
 OpenMP on the host cores) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep] [--no-overlap]
 """
 import argparse
 import ctypes
@@ -30,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sycl-fft_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+COPY_BYTES = 256 << 20  # copy-calibration buffer
 
 # (name, N, batch, double)
 WORKLOAD = [
@@ -97,10 +101,11 @@ def cpu_baseline(lib, sample_scale=16, target_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true", help="skip the three-stream report")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every transform of this workload fits one MI355X and "
@@ -115,7 +120,8 @@ def main():
     lib = oracle()
     stream = torch.cuda.current_stream(dev)
 
-    # ---------------- setup: plans and device-resident inputs
+    # ---------------- setup: autotuned plans (planning_mode::measure; tuning happens here,
+    # outside the timed region) and device-resident inputs
     g = torch.Generator(device=dev)
     g.manual_seed(20260001)
     subs = []
@@ -123,43 +129,39 @@ def main():
         rdt = torch.float64 if dbl else torch.float32
         x = (torch.rand((n * b, 2), generator=g, device=dev, dtype=rdt) * 2 - 1)
         x = torch.view_as_complex(x.contiguous())
-        fwd = sh.Plan([n], b, sh.FORWARD, double=dbl, normalization="orthogonal")
-        bwd = sh.Plan([n], b, sh.BACKWARD, double=dbl, normalization="orthogonal")
+        fwd = sh.Plan([n], b, sh.FORWARD, double=dbl, normalization="orthogonal", measure=True)
+        bwd = sh.Plan([n], b, sh.BACKWARD, double=dbl, normalization="orthogonal", measure=True)
         subs.append(dict(name=name, n=n, b=b, dbl=dbl, x=x, y=torch.empty_like(x),
                          z=torch.empty_like(x), fwd=fwd, bwd=bwd))
         log(f"{name}: fwd plan {fwd.describe().strip()}")
 
-    def step():
-        for s in subs:
-            s["fwd"].execute(s["x"], s["y"])
-            s["bwd"].execute(s["y"], s["z"])
+    # Timed step: the six executions back to back on one stream, each forward launch
+    # bracketed by HIP events on that stream (every forward plan of this workload is one
+    # kernel launch), so per-kernel durations are those of the kernel running alone.
+    events = {s["name"]: [] for s in subs}
 
-    # dominant kernel: the single-launch plan with the most bytes per launch
-    dom = max((s for s in subs if s["fwd"].launch_count() == 1),
-              key=lambda s: s["n"] * s["b"] * (16 if s["dbl"] else 8))
-    dom_bytes = 2 * dom["n"] * dom["b"] * (16 if dom["dbl"] else 8)
+    def step(timed=False):
+        for s in subs:
+            if timed:
+                a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                s["fwd"].execute(s["x"], s["y"])
+                b_.record(stream)
+                events[s["name"]].append((a, b_))
+            else:
+                s["fwd"].execute(s["x"], s["y"])
+            s["bwd"].execute(s["y"], s["z"])
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    ev = []
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(args.steps):
-        for s in subs:
-            if s is dom:
-                a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                s["fwd"].execute(s["x"], s["y"])
-                b_.record(stream)
-                ev.append((a, b_))
-            else:
-                s["fwd"].execute(s["x"], s["y"])
-            s["bwd"].execute(s["y"], s["z"])
+        step(timed=True)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -168,8 +170,67 @@ def main():
     step_flops = sum(2 * flops(s["n"], s["b"]) for s in subs)
     step_bytes = sum(2 * 2 * s["n"] * s["b"] * (16 if s["dbl"] else 8) for s in subs)
     value = step_flops / (ms_per_step * 1e-3) / 1e9
-    dom_ms = float(np.mean([a.elapsed_time(b_) for a, b_ in ev]))
+
+    # dominant kernel: the forward launch with the largest average duration
+    fwd_ms = {name: float(np.mean([a.elapsed_time(b_) for a, b_ in ev])) for name, ev in events.items()}
+    dom = max((s for s in subs if s["fwd"].launch_count() == 1), key=lambda s: fwd_ms[s["name"]])
+    dom_ms = fwd_ms[dom["name"]]
+    dom_bytes = 2 * dom["n"] * dom["b"] * (16 if dom["dbl"] else 8)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    # The three sub-workloads are independent plans, so a caller may also run each on its
+    # own stream and let them overlap on the device (the reference's out-of-order SYCL
+    # queue allows the same); that step time is reported beside the metric.
+    streams = [torch.cuda.Stream(dev) for _ in subs]
+
+    def overlapped_step():
+        begin = torch.cuda.Event()
+        begin.record(stream)
+        for s, st in zip(subs, streams):
+            st.wait_event(begin)
+            s["fwd"].execute(s["x"], s["y"], stream=st)
+            s["bwd"].execute(s["y"], s["z"], stream=st)
+        for st in streams:
+            done = torch.cuda.Event()
+            done.record(st)
+            stream.wait_event(done)
+
+    overlap_ms = None
+    if not args.no_overlap:
+        for _ in range(2):
+            overlapped_step()
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            overlapped_step()
+        b_.record(stream)
+        torch.cuda.synchronize()
+        overlap_ms = a.elapsed_time(b_) / args.steps
+
+    # Copy calibration: the provider's scaled-copy kernel (a plan whose lengths are all 1)
+    # over 256 MiB in + 256 MiB out.  Under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE
+    # readings give the factor that turns the counters into bytes.
+    ncopy = COPY_BYTES // 8
+    xc = torch.zeros(ncopy, dtype=torch.complex64, device=dev)
+    yc = torch.empty_like(xc)
+    with sh.Plan([1], ncopy, sh.FORWARD) as pc:
+        for _ in range(3):
+            pc.execute(xc, yc)
+    torch.cuda.synchronize()
+    del xc, yc
+
+    # HBM traffic per launch of the dominant kernel, from the committed rocprofv3 --pmc
+    # summary of this workload (lines "<sub-workload> ... traffic_bytes=<B>"), if present
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2_roundtrip.txt")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            for row in f:
+                fields = row.split()
+                if fields and fields[0] == dom["name"]:
+                    for fld in fields[1:]:
+                        if fld.startswith("traffic_bytes="):
+                            traffic = int(fld.split("=", 1)[1])
 
     # ---------------- accuracy of each sub-workload on sampled lines vs the FP64 oracle
     acc = {}
@@ -195,7 +256,7 @@ def main():
         for lg in range(12, 21):
             n = 1 << lg
             b = (buf // 8) // n
-            p = sh.Plan([n], b, sh.FORWARD)
+            p = sh.Plan([n], b, sh.FORWARD, measure=True)
             for _ in range(2):
                 p.execute(xs, ys)
             a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -239,11 +300,16 @@ def main():
         "config": {"workload": "c2c fwd+bwd orthogonal: f64 N=2520 B=1024, f64 N=6561 B=512, "
                                "f32 N=10007 B=128 (Bluestein)",
                    "hbm_frac_algorithmic": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "three_streams_ms_per_step": round(overlap_ms, 4) if overlap_ms else None,
+                   "three_streams_gflops": round(step_flops / (overlap_ms * 1e-3) / 1e9, 2) if overlap_ms else None,
+                   "fwd_launch_ms": {k: round(v, 4) for k, v in fwd_ms.items()},
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"tile_fft {dom['name']} forward (1 launch)",
-                     "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4)},
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "scaled by the copy-kernel factor)"},
         "cpu_baseline": cpu,
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
