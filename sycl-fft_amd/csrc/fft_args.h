@@ -30,6 +30,10 @@ struct sfh_pass
     uint32_t ns;
     uint32_t twstride;
     sfh_fastdiv ns_div;
+    // LDS layout of the tile this pass reads (pad_in) and writes (pad_out): 0 = element e
+    // at slot e, 1 = one pad slot after every 32 elements, e at e + e/32
+    uint32_t pad_in;
+    uint32_t pad_out;
 };
 
 // Compile-time fusion modes (template parameter of the kernel).
@@ -45,9 +49,6 @@ enum sfh_mode : int
 enum sfh_flags : uint32_t
 {
     SFH_CONJ_IN = 1u << 0,   // conjugate every loaded element (backward transforms)
-    SFH_CONJ_POST = 1u << 1, // conjugate after the last FFT, before POST (undoes conj-FFT-conj)
-    SFH_CONJ_OUT = 1u << 2,  // conjugate every stored element
-    SFH_SCALE = 1u << 3,     // multiply stored element by `scale`
     SFH_WFAST = 1u << 4,     // lane mapping: consecutive lanes take consecutive columns
     SFH_LOAD_LDS = 1u << 5,  // load through a coalescing LDS loop instead of into pass 0
     SFH_STORE_LDS = 1u << 6  // store through a coalescing LDS loop instead of from the last pass
@@ -55,8 +56,12 @@ enum sfh_flags : uint32_t
 
 // Load pipeline:  read -> CONJ_IN -> [PRE: zero if p >= nvalid_in, else * pre[p]]
 // Mid pipeline:   * mid[s] -> conjugate -> FFT #2
-// Store pipeline: [T4] -> CONJ_POST -> [POST: skip if q >= nvalid_out, else * post[q]]
-//                 -> SCALE -> CONJ_OUT -> write
+// Store pipeline: [T4] -> [POST: skip if q >= nvalid_out, else * post[q]]
+//                 -> (re * out_sx, im * out_sy) -> write
+// The planner folds conjugations and the normalization scale into (out_sx, out_sy): a
+// conjugation is out_sy = -out_sx; a conjugation before POST (the Bluestein inverse FFT's
+// conj-FFT-conj) is folded in with a conjugated post table, conj(v) p = conj(v conj(p)).
+
 // with p = j * pre_jmul + col * pre_cmul, s = k * mid_kmul + col * mid_cmul,
 //      q = k * post_kmul + col * post_cmul, T4 exponent = col * k,
 //      col = (wg / col_div) % col_mod + blo * col_bmul, wg = global column index.
@@ -105,7 +110,8 @@ struct sfh_tile_args
     uint32_t tw_bits, tw_entries; // pass-twiddle tables (lo has 2^tw_bits entries)
     uint32_t t4_bits, t4_entries; // four-step tables (T0, T1 have 2^t4_bits entries)
     uint32_t lds_tab;             // LDS element offset of the tables (after the W x LP tile)
+    uint32_t pad_store;           // LDS layout of the tile the STORE_LDS loop reads
     uint32_t lds_bytes;           // dynamic LDS of the launch
-    double scale;
+    double out_sx, out_sy; // store epilogue: re * out_sx, im * out_sy
     sfh_pass pass[SFH_MAX_PASSES];
 };

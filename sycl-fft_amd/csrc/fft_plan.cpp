@@ -205,6 +205,9 @@ struct launch
     uint32_t grid{};                // workgroups launched
     // table offsets into the plan's table block (patched into args at plan time)
     int64_t tw_off{-1}, pre_off{-1}, mid_off{-1}, post_off{-1};
+    // the store conjugates before POST (Bluestein's inverse FFT, conj-FFT-conj): folded into
+    // the epilogue sign with a conjugated post table
+    bool conj_post{false};
     std::string desc;
 };
 
@@ -311,9 +314,19 @@ static tw_layout twiddle_layout(uint64_t L, uint64_t N)
     return t;
 }
 
-static uint64_t lds_bytes_of(const plan &p, uint32_t L, uint32_t W, const tw_layout &t, uint32_t *tab_off)
+// LDS pitch of a tile column: L slots, plus one pad slot per 32 elements when the padded
+// layout is used, plus one slot when several columns share the tile so that lanes on
+// different columns at the same element hit different banks.
+static uint32_t lds_pitch(uint32_t L, uint32_t W, bool pad)
 {
-    const uint64_t pitch = W > 1 ? L + 1 : L;
+    const uint32_t slots = pad ? L + (L + 31u) / 32u : L;
+    return W > 1 ? slots + 1 : slots;
+}
+
+static uint64_t lds_bytes_of(const plan &p, uint32_t L, uint32_t W, const tw_layout &t, uint32_t *tab_off,
+                             bool pad = false)
+{
+    const uint64_t pitch = lds_pitch(L, W, pad);
     const uint64_t tab = (pitch * W + 1) & ~uint64_t(1);
     if (tab_off) {
         *tab_off = static_cast<uint32_t>(tab);
@@ -323,7 +336,7 @@ static uint64_t lds_bytes_of(const plan &p, uint32_t L, uint32_t W, const tw_lay
 
 static constexpr uint64_t k_lds_limit = 163840; // 160 KiB of LDS per workgroup on gfx950
 
-// Smallest variant holding W columns of length L (pitch L+1 when W > 1) plus the tables.
+// Smallest variant holding W columns of length L (at the LDS pitch) plus the tables.
 static int pick_variant(const plan &p, uint32_t L, uint32_t W, const tw_layout &t)
 {
     int best = -1;
@@ -513,6 +526,140 @@ static void check_span(const geometry &g)
     }
 }
 
+// LDS bank model: cycles one wave's access takes when lane l touches `esize` bytes at
+// element slot slot[l] (-1 = inactive), with 64 banks of 4 bytes; lanes reading the same
+// dword share it.  Ideal is esize / 4 cycles for 64 active lanes.
+static uint32_t lds_cycles(const int64_t *slot, uint32_t esize)
+{
+    uint32_t words[64 * 4];
+    uint32_t n = 0;
+    for (int l = 0; l < 64; ++l) {
+        if (slot[l] < 0) {
+            continue;
+        }
+        for (uint32_t d = 0; d < esize / 4; ++d) {
+            words[n++] = static_cast<uint32_t>(slot[l] * (esize / 4) + d);
+        }
+    }
+    std::sort(words, words + n);
+    n = static_cast<uint32_t>(std::unique(words, words + n) - words);
+    uint32_t per_bank[64] = {};
+    uint32_t worst = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        worst = std::max(worst, ++per_bank[words[k] % 64]);
+    }
+    return worst;
+}
+
+// Bank-model cost (cycles of the first wave) of the LDS traffic across one pass boundary:
+// the writer's Stockham stores (radix wr at ns, or FFT #2's first pass at ns = 1 with
+// writer_mid) and the next pass's reads (radix rr; 0 = no LDS reader), with the tile in
+// layout `pad` at pitch LP.
+static uint64_t boundary_cost(const sfh_tile_args &a, uint32_t esize, uint32_t wr, uint32_t wns, bool writer_mid,
+                              uint32_t rr, bool pad, uint32_t LP)
+{
+    const uint32_t L = a.L, W = a.W;
+    uint32_t lane_w[64], lane_i[64];
+    for (uint32_t l = 0; l < 64; ++l) {
+        if (W == 1) {
+            lane_w[l] = 0;
+            lane_i[l] = l;
+        } else if (a.flags & SFH_WFAST) {
+            lane_w[l] = l & (W - 1);
+            lane_i[l] = l >> a.logW;
+        } else {
+            lane_w[l] = l >> a.logtpc;
+            lane_i[l] = l & (a.tpc - 1);
+        }
+    }
+    const auto slot = [pad](uint64_t e) { return static_cast<int64_t>(pad ? e + (e >> 5) : e); };
+    uint64_t cost = 0;
+    int64_t addr[64];
+    const uint32_t mw = L / wr;
+    for (uint32_t t = 0; t < wr; ++t) {
+        for (uint32_t l = 0; l < 64; ++l) {
+            const uint32_t i = lane_i[l];
+            if (i >= mw) {
+                addr[l] = -1;
+                continue;
+            }
+            uint64_t e;
+            if (writer_mid) {
+                e = static_cast<uint64_t>(i) * wr + t;
+            } else {
+                const uint64_t kq = i / wns;
+                e = kq * wns * wr + (i - kq * wns) + static_cast<uint64_t>(t) * wns;
+            }
+            addr[l] = static_cast<int64_t>(lane_w[l]) * LP + slot(e);
+        }
+        cost += lds_cycles(addr, esize);
+    }
+    if (rr) {
+        const uint32_t mr = L / rr;
+        for (uint32_t t = 0; t < rr; ++t) {
+            for (uint32_t l = 0; l < 64; ++l) {
+                const uint32_t i = lane_i[l];
+                addr[l] = i < mr ? static_cast<int64_t>(lane_w[l]) * LP + slot(i + static_cast<uint64_t>(t) * mr) : -1;
+            }
+            cost += lds_cycles(addr, esize);
+        }
+    }
+    return cost;
+}
+
+// LDS layout of every tile state of the launch's program, per pass boundary: layout 0 or
+// the padded layout, by the bank model of the boundary's writes and reads.  The padded
+// layout is eligible where both sides stay linear in t (writer: power-of-two radix and
+// ns; reader: m = L/R a multiple of 32 or a power of two, or the per-element STORE_LDS
+// loop).  Program step
+// s runs pass[s] for FFT #1 and pass[s + 1] for FFT #2, whose pass 0 is fused into step
+// np - 1 (which therefore writes with FFT #2's first radix at ns = 1).  Two candidates
+// are compared: every boundary in layout 0 at the plain pitch, and the padded pitch with
+// each boundary in its cheaper layout; pads are dropped when the padded tile does not fit.
+static void set_lds_layout(const plan &p, launch &l, const tw_layout &tl, bool second_fft)
+{
+    sfh_tile_args &a = l.args;
+    const uint32_t np = a.npass;
+    const uint32_t total = second_fft ? 2 * np - 1 : np;
+    const bool store_lds = (a.flags & SFH_STORE_LDS) != 0;
+    const auto pidx = [np](uint32_t s) { return s < np ? s : s + 1; };
+    const uint32_t es = static_cast<uint32_t>(p.esize);
+    const uint32_t lp0 = lds_pitch(a.L, a.W, false), lp1 = lds_pitch(a.L, a.W, true);
+    const bool pad_fits = lds_bytes_of(p, a.L, a.W, tl, nullptr, true) <= k_lds_limit;
+    uint64_t cost0 = 0, cost1 = 0;
+    std::vector<uint32_t> choice(total, 0);
+    for (uint32_t s = 0; s < total; ++s) {
+        const bool last = s + 1 == total;
+        if (last && !store_lds) {
+            continue; // the last pass writes HBM
+        }
+        const sfh_pass &ps = a.pass[pidx(s)];
+        const bool mid = second_fft && s + 1 == np;
+        const uint32_t wr = mid ? a.pass[np].radix : ps.radix;
+        const uint32_t wns = mid ? 1u : ps.ns;
+        const uint32_t rr = last ? 0u : a.pass[pidx(s + 1)].radix;
+        const uint64_t c0 = boundary_cost(a, es, wr, wns, mid, rr, false, lp0);
+        cost0 += c0;
+        const bool eligible =
+            is_pow2(wr) && is_pow2(wns) && (last || (a.L / rr) % 32 == 0 || is_pow2(a.L / rr));
+        const uint64_t c1 = boundary_cost(a, es, wr, wns, mid, rr, false, lp1);
+        const uint64_t c1p = eligible ? boundary_cost(a, es, wr, wns, mid, rr, true, lp1) : c1;
+        choice[s] = c1p < c1 ? 1u : 0u;
+        cost1 += std::min(c1, c1p);
+    }
+    const bool use_pad = pad_fits && cost1 < cost0;
+    uint32_t prev = 0; // pass 0 reads HBM, or LDS filled in layout 0 by the load loop
+    for (uint32_t s = 0; s < total; ++s) {
+        sfh_pass &ps = a.pass[pidx(s)];
+        ps.pad_in = prev;
+        ps.pad_out = use_pad ? choice[s] : 0u;
+        prev = ps.pad_out;
+    }
+    a.pad_store = store_lds ? prev : 0u;
+    a.LP = use_pad ? lp1 : lp0;
+    a.lds_bytes = static_cast<uint32_t>(lds_bytes_of(p, a.L, a.W, tl, &a.lds_tab, use_pad));
+}
+
 static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
                           bool second_fft, const std::string &what, uint64_t t4n = 0)
 {
@@ -535,7 +682,6 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
     set_passes(l, g.L, second_fft, static_cast<uint32_t>(v.maxr));
     a.W = g.W;
     a.logW = ilog2(g.W);
-    a.LP = g.W > 1 ? g.L + 1 : g.L;
     a.tpc = static_cast<uint32_t>(v.nt) / g.W;
     a.logtpc = ilog2(a.tpc);
     a.wcount = g.wcount;
@@ -553,7 +699,8 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
     a.col_div = make_fastdiv(static_cast<uint32_t>(g.col_div));
     a.col_mod = make_fastdiv(static_cast<uint32_t>(g.col_mod));
     a.col_bmul = static_cast<uint32_t>(g.col_bmul);
-    a.scale = 1.0;
+    a.out_sx = 1.0;
+    a.out_sy = 1.0;
     // lane mapping: consecutive lanes on consecutive columns when the source is
     // column-contiguous; otherwise along the column.
     const bool in_wc = (g.in_w == 1 && g.W > 1);
@@ -570,13 +717,13 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
     a.tw_entries = tl.tw_entries;
     a.t4_bits = tl.t4_bits;
     a.t4_entries = tl.t4_entries;
-    a.lds_bytes = static_cast<uint32_t>(lds_bytes_of(p, g.L, g.W, tl, &a.lds_tab));
+    set_lds_layout(p, l, tl, second_fft);
     l.tw_off = twiddle_tables(p, g.L, t4n, tl);
     l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
     char buf[256];
     std::snprintf(
-        buf, sizeof(buf), "%s L=%u W=%u tiles=%llu nt=%d e=%d mode=%d flags=0x%x lds=%u", what.c_str(),
-        g.L, g.W, static_cast<unsigned long long>(a.ntiles), v.nt, v.e, mode, a.flags, a.lds_bytes);
+        buf, sizeof(buf), "%s L=%u W=%u tiles=%llu nt=%d e=%d mode=%d flags=0x%x lds=%u lp=%u", what.c_str(),
+        g.L, g.W, static_cast<unsigned long long>(a.ntiles), v.nt, v.e, mode, a.flags, a.lds_bytes, a.LP);
     l.desc = buf;
     return l;
 }
@@ -814,6 +961,12 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
     }
     const std::string nkey = std::to_string(n) + "_" + std::to_string(M);
     const int64_t chirp_off = table(p, "chirp" + nkey, chirp);
+    // post table of the conj-before-POST stores: conj(v) c = conj(v conj(c))
+    std::vector<cxl> chirpc(chirp);
+    for (auto &v : chirpc) {
+        v.im = -v.im;
+    }
+    const int64_t chirpc_off = table(p, "chirpc" + nkey, chirpc);
 
     if (fused) {
         geometry g;
@@ -847,9 +1000,9 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
         a.post_kmul = 1;
         a.post_cmul = 0;
         a.nvalid_out = static_cast<uint32_t>(n);
-        a.flags |= SFH_CONJ_POST;
+        l.conj_post = true;
         l.pre_off = chirp_off;
-        l.post_off = chirp_off;
+        l.post_off = chirpc_off;
         l.mid_off = table(p, "spec" + nkey, spec);
         p.launches.push_back(l);
         return;
@@ -943,8 +1096,8 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
         l.args.post_kmul = static_cast<uint32_t>(m2);
         l.args.post_cmul = 1;
         l.args.nvalid_out = static_cast<uint32_t>(n);
-        l.args.flags |= SFH_CONJ_POST;
-        l.post_off = chirp_off;
+        l.conj_post = true;
+        l.post_off = chirpc_off;
         l.dst_off = off;
         p.launches.push_back(l);
     }
@@ -1140,18 +1293,15 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
                 p->launches[i].args.flags |= SFH_CONJ_IN;
             }
         }
-        for (size_t i = last_begin; i < p->launches.size(); ++i) {
+        // store epilogue (re * out_sx, im * out_sy) of every launch: the scale and the output
+        // conjugation on final writes, and the conj-before-POST sign
+        for (size_t i = 0; i < p->launches.size(); ++i) {
             launch &l = p->launches[i];
-            if (l.dst != BUF_OUT) {
-                continue;
-            }
-            if (cfg->direction == 1) {
-                l.args.flags |= SFH_CONJ_OUT;
-            }
-            if (p->scale != 1.0) {
-                l.args.flags |= SFH_SCALE;
-                l.args.scale = p->scale;
-            }
+            const bool final_write = i >= last_begin && l.dst == BUF_OUT;
+            const bool conj_out = final_write && cfg->direction == 1;
+            const double sc = final_write ? p->scale : 1.0;
+            l.args.out_sx = sc;
+            l.args.out_sy = (conj_out != l.conj_post) ? -sc : sc;
         }
     }
 

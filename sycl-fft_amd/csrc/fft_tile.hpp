@@ -77,6 +77,19 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, sfh_fastdiv f)
     return (__umulhi(n, f.m) + n) >> f.s;
 }
 
+// LDS slot of element e of a tile column.  Layout 0: slot e.  Layout 1 (pad): one pad
+// slot after every 32 elements, slot e + e/32.  Stockham writes of an even radix at small
+// ns go to power-of-two strides (e.g. stride 16 after a radix-16 pass 0), which in layout
+// 0 put all 64 lanes of a wave on a few banks; the pad spreads them over every bank.  The
+// planner picks the layout per pass boundary such that both sides stay linear in the
+// butterfly input/output index t (writer: power-of-two radix and ns; reader: m = L/R a
+// multiple of 32 or a power of two), so every LDS access costs one add of a wave-uniform
+// offset to a per-butterfly base.
+__device__ __forceinline__ uint32_t lds_slot(uint32_t e, uint32_t pad)
+{
+    return e + (pad ? (e >> 5) : 0u);
+}
+
 // ---------------------------------------------------------------------------------
 // Forward DFT butterflies, y[k] = sum_t x[t] exp(-2 pi i t k / R), in place on v[0..R).
 // ---------------------------------------------------------------------------------
@@ -408,17 +421,27 @@ __device__ __forceinline__ void twiddle_powers(cx<T> *w, uint32_t e, Lookup lk)
     }
 }
 
-// T4 = 1: apply the four-step twiddle W_N^(col k) inside (else the caller already did).
-template <typename T, int MODE, int T4 = 1>
+// Store epilogue factors (re * sx, im * sy): conjugations and the normalization scale.
+template <typename T>
+struct epilogue
+{
+    T sx, sy;
+    __device__ __forceinline__ explicit epilogue(const sfh_tile_args &a)
+        : sx(static_cast<T>(a.out_sx)), sy(static_cast<T>(a.out_sy))
+    {}
+    __device__ __forceinline__ cx<T> operator()(cx<T> v) const
+    {
+        return {v.x * sx, v.y * sy};
+    }
+};
+
+// Store pipeline for one element (the coalescing STORE_LDS loop): [T4] -> [POST] -> epilogue.
+template <typename T, int MODE>
 __device__ __forceinline__ void store_one(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *colp, uint32_t k, uint32_t col, cx<T> v)
 {
-    const uint32_t f = a.flags;
-    if constexpr ((MODE & SFH_M_T4) && T4) {
+    if constexpr (MODE & SFH_M_T4) {
         v = cmul(v, t4_twiddle(a, c.t4, col * k));
-    }
-    if (f & SFH_CONJ_POST) {
-        v.y = -v.y;
     }
     if constexpr (MODE & SFH_M_POST) {
         const uint32_t q = k * a.post_kmul + col * a.post_cmul;
@@ -427,20 +450,16 @@ __device__ __forceinline__ void store_one(
         }
         v = cmul(v, static_cast<const cx<T> *>(a.post)[q]);
     }
-    if (f & SFH_SCALE) {
-        v = cscale(v, static_cast<T>(a.scale));
-    }
-    if (f & SFH_CONJ_OUT) {
-        v.y = -v.y;
-    }
-    colp[static_cast<int32_t>(k) * a.out_k] = v;
+    colp[static_cast<int32_t>(k) * a.out_k] = epilogue<T>(a)(v);
 }
 
 // ---------------------------------------------------------------------------------
 // One Stockham pass.  Source/destination of the butterfly data:
 //   IN_GLOBAL  : pass 0 reads x[i + t L/R] of its column straight from HBM (pre-ops)
-//   IN_LDS     : read lds[w LP + i + t L/R], twiddle by W_{ns R}^{(i mod ns) t}
-//   OUT_LDS    : Stockham write lds[w LP + (i/ns) ns R + i%ns + t ns]
+//   IN_LDS     : read element i + t L/R of the LDS column, twiddle by W_{ns R}^{(i mod ns) t}
+//   OUT_LDS    : Stockham write of element (i/ns) ns R + i%ns + t ns
+//   (element e of column w lives at lds[w LP + lds_slot(e, layout)]; the load loop of
+//   LOAD_LDS writes layout 0, which pass 0 then reads)
 //   OUT_GLOBAL : last pass (ns R == L, i < ns): output k = i + t L/R goes to HBM
 //   OUT_MID    : last pass of FFT #1: spectrum product + conjugation in registers, then
 //                pass 0 of FFT #2 (radix R, ns = 1, same input positions), then OUT_LDS
@@ -513,17 +532,74 @@ __device__ __forceinline__ void run_pass(
     cx<T> v[NB][R];
 
     if (in_mode == IN_GLOBAL) {
+        // Input t of butterfly i is element i + t m of the column: one 64-bit base per
+        // butterfly and a wave-uniform step per t, so an element costs one address add.
+        const int64_t tstep = static_cast<int64_t>(m) * a.in_j;
+        if constexpr (MODE & SFH_M_PRE) {
+            // zero padding: element j exists iff p = j pre_jmul + col pre_cmul < nvalid_in;
+            // the chirp value pre[p] is loaded beside it so both round trips overlap
+            const cx<T> *pre = static_cast<const cx<T> *>(a.pre);
+            const uint32_t pstep = m * a.pre_jmul;
+            cx<T> w[NB][R];
 #pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
-            if (wvalid && i < m) {
+            for (int q = 0; q < NB; ++q) {
+                const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+                if (wvalid && i < m) {
+                    const cx<T> *bq = c.src + static_cast<int64_t>(i) * a.in_j;
+                    const uint32_t pq = i * a.pre_jmul + c.col * a.pre_cmul;
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const uint32_t pp = pq + static_cast<uint32_t>(t) * pstep;
+                        if (pp < a.nvalid_in) {
+                            v[q][t] = bq[t * tstep];
+                            w[q][t] = pre[pp];
+                        } else {
+                            v[q][t] = {T(0), T(0)};
+                            w[q][t] = {T(0), T(0)};
+                        }
+                    }
+                }
+            }
+            after_loads();
+            if (a.flags & SFH_CONJ_IN) { // wave-uniform: a scalar branch around the loop
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        v[q][t].y = -v[q][t].y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    v[q][t] = load_one<T, MODE>(a, c.src, i + t * m, c.col);
+                    v[q][t] = cmul(v[q][t], w[q][t]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+                if (wvalid && i < m) {
+                    const cx<T> *bq = c.src + static_cast<int64_t>(i) * a.in_j;
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        v[q][t] = bq[t * tstep];
+                    }
+                }
+            }
+            after_loads();
+            if (a.flags & SFH_CONJ_IN) { // wave-uniform: a scalar branch around the loop
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        v[q][t].y = -v[q][t].y;
+                    }
                 }
             }
         }
-        after_loads();
     }
 
 #pragma unroll
@@ -538,9 +614,13 @@ __device__ __forceinline__ void run_pass(
             } else if (in_mode == IN_GLOBAL) {
                 // loaded above
             } else {
+                // m is a multiple of 32 or a power of two <= 32 whenever pad_in is set, so
+                // slot(i + t m) = slot(i) + t m + (t m) / 32 for every i < m
+                const uint32_t gi = lds_slot(i, p.pad_in);
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    v[q][t] = ldsc[i + t * m];
+                    const uint32_t tm = static_cast<uint32_t>(t) * m;
+                    v[q][t] = ldsc[gi + tm + (p.pad_in ? (tm >> 5) : 0u)];
                 }
                 if (p.ns > 1) {
                     const uint32_t kq = fdiv(i, p.ns_div);
@@ -561,11 +641,14 @@ __device__ __forceinline__ void run_pass(
     }
 
     if (out_mode == OUT_GLOBAL) {
+        // Output t of butterfly i is element k = i + t m of the column:
+        // [T4] -> [POST] -> epilogue -> write, one 64-bit base per butterfly and a
+        // wave-uniform step per t.
+        if constexpr (MODE & SFH_M_T4) {
 #pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
-            if (wvalid && i < m) {
-                if constexpr (MODE & SFH_M_T4) {
+            for (int q = 0; q < NB; ++q) {
+                const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+                if (wvalid && i < m) {
                     // W_N^(col (i + t m)) = W_N^(col i) (W_N^(col m))^t
                     const auto lk = [&](uint32_t x) { return t4_twiddle(a, c.t4, x); };
                     const cx<T> base = lk(c.col * i);
@@ -573,12 +656,36 @@ __device__ __forceinline__ void run_pass(
                     twiddle_powers<T, R>(w, c.col * m, lk);
 #pragma unroll
                     for (int t = 0; t < R; ++t) {
-                        store_one<T, MODE, 0>(a, c, c.dst, i + t * m, c.col, cmul(v[q][t], cmul(base, w[t])));
+                        v[q][t] = cmul(v[q][t], cmul(base, w[t]));
+                    }
+                }
+                SFH_SCHED_BARRIER();
+            }
+        }
+        const epilogue<T> ep(a);
+        const int64_t tstep = static_cast<int64_t>(m) * a.out_k;
+        // POST: output k exists iff pq = k post_kmul + col post_cmul < nvalid_out and is
+        // multiplied by post[pq]
+        const uint32_t qstep = m * a.post_kmul;
+        const cx<T> *post = static_cast<const cx<T> *>(a.post);
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+            if (wvalid && i < m) {
+                cx<T> *bq = c.dst + static_cast<int64_t>(i) * a.out_k;
+                if constexpr (MODE & SFH_M_POST) {
+                    const uint32_t qq = i * a.post_kmul + c.col * a.post_cmul;
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const uint32_t pq = qq + static_cast<uint32_t>(t) * qstep;
+                        if (pq < a.nvalid_out) {
+                            bq[t * tstep] = ep(cmul(v[q][t], post[pq]));
+                        }
                     }
                 } else {
 #pragma unroll
                     for (int t = 0; t < R; ++t) {
-                        store_one<T, MODE>(a, c, c.dst, i + t * m, c.col, v[q][t]);
+                        bq[t * tstep] = ep(v[q][t]);
                     }
                 }
             }
@@ -590,14 +697,15 @@ __device__ __forceinline__ void run_pass(
     if constexpr (MODE & SFH_M_MID) {
         if (out_mode == OUT_MID) {
             const cx<T> *mid = static_cast<const cx<T> *>(a.mid);
+            const uint32_t sstep = m * a.mid_kmul;
 #pragma unroll
             for (int q = 0; q < NB; ++q) {
                 const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
                 if (wvalid && i < m) {
+                    const uint32_t sq = i * a.mid_kmul + c.col * a.mid_cmul;
 #pragma unroll
                     for (int t = 0; t < R; ++t) {
-                        const uint32_t s = (i + t * m) * a.mid_kmul + c.col * a.mid_cmul;
-                        v[q][t] = cconj(cmul(v[q][t], mid[s]));
+                        v[q][t] = cconj(cmul(v[q][t], mid[sq + static_cast<uint32_t>(t) * sstep]));
                     }
                     dft<T, R>::run(v[q]);
                 }
@@ -620,9 +728,13 @@ __device__ __forceinline__ void run_pass(
                 base = kq * p.ns * (R - 1) + i; // kq ns R + (i - kq ns)
                 stride = p.ns;
             }
+            // pad_out is set only for power-of-two R and ns, where element base + t ns stays
+            // inside base's 32-block run: slot(base + t ns) = slot(base) + t ns + (t ns) / 32
+            const uint32_t gb = lds_slot(base, p.pad_out);
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                ldsc[base + t * stride] = v[q][t];
+                const uint32_t ts = static_cast<uint32_t>(t) * stride;
+                ldsc[gb + ts + (p.pad_out ? (ts >> 5) : 0u)] = v[q][t];
             }
         }
     }
@@ -814,7 +926,7 @@ __device__ __forceinline__ void store_via_lds(const sfh_tile_args &a, const tile
             const uint32_t col =
                 (MODE & (SFH_M_T4 | SFH_M_POST)) ? col_of(a, c.w0, c.blo, w) : 0u;
             store_one<T, MODE>(
-                a, c, c.tdst + static_cast<int32_t>(w) * a.out_w, k, col, lds[w * a.LP + k]);
+                a, c, c.tdst + static_cast<int32_t>(w) * a.out_w, k, col, lds[w * a.LP + lds_slot(k, a.pad_store)]);
         }
     }
 }
