@@ -139,6 +139,18 @@ SYCLFFT_HIP_API int syclfft_hip_plan_execute(
     void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
     void *done, char *err, size_t cap);
 
+/* Enqueue `count` transforms of the plan's shape (1 <= count <= the plan's batch count)
+ * given by host arrays of device pointers: transform k reads ins[k] and writes outs[k]
+ * (ins[k] == outs[k] for in-place plans).  Same ordering, dependency and completion
+ * semantics as syclfft_hip_plan_execute; the pointer arrays are copied before the call
+ * returns.  One plan then serves any number of independent buffers up to its batch count
+ * in one set of launches (the serving loop batches requests this way).  No reference
+ * counterpart: the reference executes one contiguous batch per call
+ * (src/sycl/sycl.cpp:615-776). */
+SYCLFFT_HIP_API int syclfft_hip_plan_execute_many(
+    void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *const *deps,
+    size_t ndeps, void *done, char *err, size_t cap);
+
 /* Human-readable kernel sequence of a plan (one line per launch) for tracing and tests;
  * returns the full length (like snprintf). No reference counterpart. */
 SYCLFFT_HIP_API size_t syclfft_hip_plan_describe(const void *plan, char *buf, size_t cap);

@@ -54,6 +54,14 @@ enum sfh_flags : uint32_t
     SFH_STORE_LDS = 1u << 6  // store through a coalescing LDS loop instead of from the last pass
 };
 
+// Dimension of a launch's tile grid that enumerates transforms (for unit tables).
+enum sfh_unit_dim : uint32_t
+{
+    SFH_UNIT_COL = 1, // column index w0 + w (contiguous 1-D axes: one column per line)
+    SFH_UNIT_BLO = 2, // low batch index blo
+    SFH_UNIT_BHI = 3  // high batch index bhi
+};
+
 // Load pipeline:  read -> CONJ_IN -> [PRE: zero if p >= nvalid_in, else * pre[p]]
 // Mid pipeline:   * mid[s] -> conjugate -> FFT #2
 // Store pipeline: [T4] -> [POST: skip if q >= nvalid_out, else * post[q]]
@@ -80,6 +88,12 @@ struct sfh_tile_args
     const void *mid;  // mid-op table (spectrum with 1/M folded in)
     const void *post; // post-op table
 
+    // Per-transform unit tables (execution over pointer arrays): when set, the base of unit
+    // u of the launch's transform dimension (unit_dim) on that side is units[u] instead of
+    // in/out + u * stride.  Null for plan-owned scratch and for ordinary executions.
+    const void *const *in_units;
+    void *const *out_units;
+
     // tile index g -> (wt, blo, bhi): wt fastest, then blo, then bhi
     uint64_t wcount; // extent of the column dimension (ragged last tile is masked)
     uint64_t nwt;    // number of column tiles = ceil(wcount / W)
@@ -101,6 +115,7 @@ struct sfh_tile_args
     uint8_t rcount[8];
     uint8_t rcount2[8];
     uint32_t flags;
+    uint32_t unit_dim; // SFH_UNIT_*: the dimension that enumerates transforms
     sfh_fastdiv L_div;
     sfh_fastdiv col_div, col_mod;
     uint32_t col_bmul;

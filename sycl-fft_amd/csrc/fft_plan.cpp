@@ -208,6 +208,11 @@ struct launch
     // the store conjugates before POST (Bluestein's inverse FFT, conj-FFT-conj): folded into
     // the epilogue sign with a conjugated post table
     bool conj_post{false};
+    // transform dimension of the tile grid (SFH_UNIT_*), its extent at the plan's batch
+    // count, and the element strides between its units on each side
+    uint32_t unit_dim{0};
+    uint64_t unit_extent{0};
+    int64_t unit_in_stride{0}, unit_out_stride{0};
     std::string desc;
 };
 
@@ -264,6 +269,16 @@ struct plan
     int nvariants{};
     int copy_only{}; // all lengths 1: a scaled copy
     int cap_limit{};  // largest tile (complex elements) the planner may use; 0 = no limit
+    // unit-table ring for executions over pointer arrays (allocated on first use): per slot
+    // a device table block, its pinned host staging copy, and the event after which the
+    // slot's last execution no longer reads it
+    static constexpr int k_unit_slots = 4;
+    uint64_t unit_bytes{};
+    void *unit_dev[k_unit_slots]{};
+    void *unit_host[k_unit_slots]{};
+    hipEvent_t unit_free[k_unit_slots]{};
+    bool unit_used[k_unit_slots]{};
+    int unit_next{};
 };
 
 static bool variant_allowed(const plan &p, int i)
@@ -511,6 +526,7 @@ struct geometry
     int64_t in_w{}, in_j{}, in_blo{}, in_bhi{};
     int64_t out_w{}, out_k{}, out_blo{}, out_bhi{};
     uint64_t col_div{1}, col_mod{1}, col_bmul{0};
+    uint32_t unit_dim{0}; // SFH_UNIT_*: the dimension that enumerates the axis' outer lines
 };
 
 static void check_span(const geometry &g)
@@ -718,6 +734,21 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
     a.t4_bits = tl.t4_bits;
     a.t4_entries = tl.t4_entries;
     set_lds_layout(p, l, tl, second_fft);
+    l.unit_dim = g.unit_dim;
+    if (g.unit_dim == SFH_UNIT_COL) {
+        l.unit_extent = g.wcount;
+        l.unit_in_stride = g.in_w;
+        l.unit_out_stride = g.out_w;
+    } else if (g.unit_dim == SFH_UNIT_BLO) {
+        l.unit_extent = g.nblo;
+        l.unit_in_stride = g.in_blo;
+        l.unit_out_stride = g.out_blo;
+    } else if (g.unit_dim == SFH_UNIT_BHI) {
+        l.unit_extent = g.nbhi;
+        l.unit_in_stride = g.in_bhi;
+        l.unit_out_stride = g.out_bhi;
+    }
+    a.unit_dim = g.unit_dim;
     l.tw_off = twiddle_tables(p, g.L, t4n, tl);
     l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
     char buf[256];
@@ -826,12 +857,14 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
             if (inner == 1) {
                 g.W = choose_w(p, n, outer, false);
                 g.wcount = outer;
+                g.unit_dim = SFH_UNIT_COL;
                 g.in_w = g.out_w = static_cast<int64_t>(n);
                 g.in_j = g.out_k = 1;
             } else {
                 g.W = choose_w(p, n, inner, true);
                 g.wcount = inner;
                 g.nblo = outer;
+                g.unit_dim = SFH_UNIT_BLO;
                 g.in_w = g.out_w = 1;
                 g.in_j = g.out_k = static_cast<int64_t>(inner);
                 g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
@@ -876,6 +909,7 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
             g.wcount = n2 * inner;
             g.W = choose_w(p, n1, g.wcount, true);
             g.nblo = outer;
+            g.unit_dim = SFH_UNIT_BLO;
             g.in_w = g.out_w = 1;
             g.in_j = g.out_k = static_cast<int64_t>(n2 * inner);
             g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
@@ -893,6 +927,7 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
                 g.wcount = n1;
                 g.W = choose_w(p, n2, n1, true);
                 g.nblo = outer;
+                g.unit_dim = SFH_UNIT_BLO;
                 g.in_w = static_cast<int64_t>(n2);
                 g.in_j = 1;
                 g.out_w = 1;
@@ -903,6 +938,7 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
                 g.W = choose_w(p, n2, inner, true);
                 g.nblo = n1;
                 g.nbhi = outer;
+                g.unit_dim = SFH_UNIT_BHI;
                 g.in_w = g.out_w = 1;
                 g.in_j = static_cast<int64_t>(inner);
                 g.out_k = static_cast<int64_t>(n1 * inner);
@@ -923,8 +959,9 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
     // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
     const uint64_t need = 2 * n - 1;
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
-    uint64_t M = p.tune.split ? 0 : smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
-    const bool fused = M != 0;
+    const uint64_t fusedM = smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+    uint64_t M = p.tune.split ? 0 : fusedM;
+    bool fused = M != 0;
     uint64_t m1 = 0, m2 = 0;
     if (!fused) {
         uint32_t wmin = min_cols(p);
@@ -942,6 +979,13 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
             wmin /= 2;
         }
         split_two(p, M, wmin, m1, m2);
+        // launch B runs two FFTs of m2 points fused into one program, which needs at least
+        // two radix passes; a split that short is only reachable by a forced split of a
+        // small M, which then stays fused
+        if (radix_plan(m2, 16).size() < 2 && fusedM) {
+            M = fusedM;
+            fused = true;
+        }
     }
     // tables
     std::vector<cxl> chirp(n), bvec(M, cxl{0.0L, 0.0L});
@@ -977,6 +1021,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
                 g.W /= 2;
             }
             g.wcount = outer;
+            g.unit_dim = SFH_UNIT_COL;
             g.in_w = g.out_w = static_cast<int64_t>(n);
             g.in_j = g.out_k = 1;
         } else {
@@ -986,6 +1031,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
             }
             g.wcount = inner;
             g.nblo = outer;
+            g.unit_dim = SFH_UNIT_BLO;
             g.in_w = g.out_w = 1;
             g.in_j = g.out_k = static_cast<int64_t>(inner);
             g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
@@ -1029,6 +1075,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
         g.wcount = m2 * inner;
         g.W = choose_w(p, m1, g.wcount, true);
         g.nblo = outer;
+        g.unit_dim = SFH_UNIT_BLO;
         g.in_w = 1;
         g.in_j = static_cast<int64_t>(m2 * inner);
         g.in_blo = static_cast<int64_t>(n * inner);
@@ -1053,6 +1100,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
             g.wcount = m1;
             g.W = choose_w(p, m2, m1, false);
             g.nblo = outer;
+            g.unit_dim = SFH_UNIT_BLO;
             g.in_w = g.out_w = static_cast<int64_t>(m2);
             g.in_j = g.out_k = 1;
             g.in_blo = g.out_blo = static_cast<int64_t>(M);
@@ -1063,6 +1111,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
             g.W = choose_w(p, m2, inner, true);
             g.nblo = m1;
             g.nbhi = outer;
+            g.unit_dim = SFH_UNIT_BHI;
             g.in_w = g.out_w = 1;
             g.in_j = g.out_k = static_cast<int64_t>(inner);
             g.in_blo = g.out_blo = static_cast<int64_t>(m2 * inner);
@@ -1084,6 +1133,7 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
         g.wcount = m2 * inner;
         g.W = choose_w(p, m1, g.wcount, true);
         g.nblo = outer;
+        g.unit_dim = SFH_UNIT_BLO;
         g.in_w = 1;
         g.in_j = static_cast<int64_t>(m2 * inner);
         g.in_blo = static_cast<int64_t>(M * inner);
@@ -1236,6 +1286,18 @@ static void release(plan *p)
     }
     if (p->tables) {
         (void)hipFree(p->tables);
+    }
+    for (int k = 0; k < plan::k_unit_slots; ++k) {
+        if (p->unit_free[k]) {
+            (void)hipEventSynchronize(p->unit_free[k]);
+            (void)hipEventDestroy(p->unit_free[k]);
+        }
+        if (p->unit_dev[k]) {
+            (void)hipFree(p->unit_dev[k]);
+        }
+        if (p->unit_host[k]) {
+            (void)hipHostFree(p->unit_host[k]);
+        }
     }
     delete p;
 }
@@ -1695,6 +1757,154 @@ static void execute_plan(
     }
 }
 
+// Execution of `count` transforms of the plan's shape given by pointer arrays: transform k
+// reads ins[k] and writes outs[k] (count <= the plan's batch count).  Every launch that
+// touches a caller buffer gets a unit table (one base pointer per unit of its transform
+// dimension) in a device slot of the plan's ring; every launch's transform dimension is cut
+// to `count` transforms, so scratch holds the first `count` transforms' intermediates.
+static void execute_units(
+    plan &p, uint64_t count, const void *const *ins, void *const *outs, hipStream_t stream, void *const *deps,
+    size_t ndeps, hipEvent_t done)
+{
+    if (p.copy_only || p.launches.empty()) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "pointer-array execution needs a plan with transform launches");
+    }
+    if (count == 0 || count > p.cfg.batch_count) {
+        throw sfh_error(
+            SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "transform count must be between 1 and the plan's batch count");
+    }
+    if (!ins || !outs) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, "pointer arrays must be non-null");
+    }
+    const uint64_t bytes = p.transform * p.esize;
+    for (uint64_t k = 0; k < count; ++k) {
+        validate_pointer(p, ins[k], bytes, "input");
+        validate_pointer(p, outs[k], bytes, "output");
+        if (p.cfg.in_place != (ins[k] == outs[k] ? 1u : 0u)) {
+            throw sfh_error(
+                SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                p.cfg.in_place ? "In-place plan must be executed with execute(inout)"
+                               : "Out-of-place plan requires distinct allocations");
+        }
+        if (!p.cfg.in_place) {
+            const auto a = reinterpret_cast<uintptr_t>(ins[k]), b = reinterpret_cast<uintptr_t>(outs[k]);
+            if (a < b + bytes && b < a + bytes) {
+                throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
+            }
+        }
+    }
+    // table layout: per launch, an input table then an output table where the side is a
+    // caller buffer; sized for the plan's batch count
+    const uint64_t B = p.cfg.batch_count;
+    uint64_t cap_entries = 0;
+    for (const auto &l : p.launches) {
+        if (l.unit_dim == 0 || l.unit_extent % B != 0 || l.src_off != 0 || l.dst_off != 0) {
+            throw sfh_error(
+                SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                "plan layout does not support pointer-array execution (chunked or unannotated launch)");
+        }
+        if (l.unit_dim == SFH_UNIT_COL && (l.args.flags & (SFH_LOAD_LDS | SFH_STORE_LDS))) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "column-unit launch with an LDS load/store loop");
+        }
+        const uint64_t per = l.unit_extent;
+        cap_entries += (l.src != BUF_SCRATCH ? per : 0) + (l.dst != BUF_SCRATCH ? per : 0);
+    }
+    std::lock_guard<std::mutex> lock(p.mu);
+    device_guard guard(p.device);
+    if (!p.unit_bytes) {
+        p.unit_bytes = std::max<uint64_t>(cap_entries, 1) * sizeof(void *);
+        for (int k = 0; k < plan::k_unit_slots; ++k) {
+            hip_check(hipMalloc(&p.unit_dev[k], p.unit_bytes), "hipMalloc(unit tables)");
+            hip_check(hipHostMalloc(&p.unit_host[k], p.unit_bytes, hipHostMallocDefault), "hipHostMalloc(unit tables)");
+            hip_check(hipEventCreateWithFlags(&p.unit_free[k], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+    const int slot = p.unit_next;
+    p.unit_next = (p.unit_next + 1) % plan::k_unit_slots;
+    if (p.unit_used[slot]) {
+        hip_check(hipEventSynchronize(p.unit_free[slot]), "hipEventSynchronize(unit slot)");
+    }
+    void **host = static_cast<void **>(p.unit_host[slot]);
+    void **dev = static_cast<void **>(p.unit_dev[slot]);
+    struct side_tables
+    {
+        const void *const *in;
+        void *const *out;
+    };
+    std::vector<side_tables> tabs(p.launches.size(), side_tables{nullptr, nullptr});
+    uint64_t pos = 0;
+    const auto fill = [&](const launch &l, buf_id b, int64_t stride) -> void ** {
+        const void *const *src = (b == BUF_IN) ? ins : reinterpret_cast<const void *const *>(outs);
+        const uint64_t units = l.unit_extent / B;
+        void **t = host + pos;
+        for (uint64_t k = 0; k < count; ++k) {
+            const unsigned char *base = static_cast<const unsigned char *>(src[k]);
+            for (uint64_t r = 0; r < units; ++r) {
+                t[k * units + r] = const_cast<unsigned char *>(base + static_cast<int64_t>(r) * stride * static_cast<int64_t>(p.esize));
+            }
+        }
+        void **d = dev + pos;
+        pos += count * units;
+        return d;
+    };
+    for (size_t i = 0; i < p.launches.size(); ++i) {
+        const launch &l = p.launches[i];
+        if (l.src != BUF_SCRATCH) {
+            tabs[i].in = fill(l, l.src, l.unit_in_stride);
+        }
+        if (l.dst != BUF_SCRATCH) {
+            tabs[i].out = fill(l, l.dst, l.unit_out_stride);
+        }
+    }
+    for (size_t i = 0; i < ndeps; ++i) {
+        if (deps[i]) {
+            hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
+        }
+    }
+    if (p.have_last) {
+        hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
+    }
+    if (pos) {
+        hip_check(hipMemcpyAsync(dev, host, pos * sizeof(void *), hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync(unit tables)");
+    }
+    for (size_t i = 0; i < p.launches.size(); ++i) {
+        const launch &l = p.launches[i];
+        sfh_tile_args a = l.args;
+        const uint64_t units = l.unit_extent / B;
+        const uint64_t ext = count * units;
+        a.in = (l.src == BUF_SCRATCH) ? p.scratch : nullptr;
+        a.out = (l.dst == BUF_SCRATCH) ? p.scratch : nullptr;
+        a.in_units = tabs[i].in;
+        a.out_units = tabs[i].out;
+        const uint64_t nbhi = a.ntiles / (a.nwt * a.nblo);
+        if (l.unit_dim == SFH_UNIT_COL) {
+            a.wcount = ext;
+            a.nwt = (ext + a.W - 1) / a.W;
+            a.ntiles = a.nwt * a.nblo * nbhi;
+        } else if (l.unit_dim == SFH_UNIT_BLO) {
+            a.nblo = ext;
+            a.ntiles = a.nwt * ext * nbhi;
+        } else {
+            a.ntiles = a.nwt * a.nblo * ext;
+        }
+        const bool persistent = l.mode == SFH_MI_PLAIN_PF || l.mode == SFH_MI_T4_PF;
+        const uint64_t grid = persistent ? std::min<uint64_t>(l.grid, a.ntiles) : a.ntiles;
+        hip_check(
+            static_cast<hipError_t>(sfh_launch_tile(
+                &p.variants[l.variant], l.mode, &a, static_cast<uint32_t>(std::min<uint64_t>(grid, 0xffffffffu)), stream)),
+            "tile kernel launch");
+    }
+    hip_check(hipEventRecord(p.unit_free[slot], stream), "hipEventRecord(unit slot)");
+    p.unit_used[slot] = true;
+    hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+    p.have_last = true;
+    if (done) {
+        hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+    }
+}
+
 static std::string describe_plan(const plan &p)
 {
     std::string s;
@@ -1807,6 +2017,34 @@ extern "C" int syclfft_hip_plan_execute(
         }
         sfh::execute_plan(
             *static_cast<sfh::plan *>(plan), in, out, static_cast<hipStream_t>(stream), deps, ndeps,
+            static_cast<hipEvent_t>(done));
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+extern "C" int syclfft_hip_plan_execute_many(
+    void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *const *deps,
+    size_t ndeps, void *done, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE, "plan handle is null");
+        }
+        if (ndeps && !deps) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "dependency list is null");
+        }
+        sfh::execute_units(
+            *static_cast<sfh::plan *>(plan), count, ins, outs, static_cast<hipStream_t>(stream), deps, ndeps,
             static_cast<hipEvent_t>(done));
         return SYCLFFT_HIP_STATUS_OK;
     } catch (const sfh_error &ex) {

@@ -1,0 +1,696 @@
+// Serving loop of the MI355X provider: continuous batching of transform requests over
+// resident plans (C ABI in include/syclfft_hip_server.h).
+//
+//   submit()      validates a request, queues it, returns a ticket
+//   dispatcher    takes the oldest waiting request, merges the other waiting requests of the
+//                 same shape into its launch (up to the per-launch transform limit), admits
+//                 the group if the HBM footprint (request buffers in flight + resident
+//                 plans) stays within the budget and a launch slot is free, picks or creates
+//                 a plan whose batch count covers the group, and enqueues one pointer-array
+//                 execution on the next stream
+//   completer     polls the launches' events, completes their requests (latency records),
+//                 returns their footprint and wakes the dispatcher and waiters
+//
+// Plans stay resident between requests (twiddle tables, chirp spectra, scratch) and are
+// evicted least recently used when a new plan would not fit the budget.  Requests of one
+// shape admitted while earlier launches run go to other streams and other plan instances,
+// so their work overlaps on the device.  No reference counterpart: the reference's only
+// concurrency model is one plan per concurrent stream of work (README.md:45-47).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "syclfft_hip_server.h"
+
+namespace sfh_srv
+{
+
+static int64_t now_ns()
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct srv_error : std::runtime_error
+{
+    int status;
+    srv_error(int s, const std::string &m)
+      : std::runtime_error(m), status(s)
+    {}
+};
+
+static void set_err(char *err, size_t cap, const std::string &msg)
+{
+    if (err && cap) {
+        const size_t n = std::min(cap - 1, msg.size());
+        std::memcpy(err, msg.data(), n);
+        err[n] = '\0';
+    }
+}
+
+struct request
+{
+    syclfft_hip_request_v1 r{};
+    uint64_t ticket{};
+    std::string key;
+    uint64_t bytes{}; // HBM footprint of its buffers
+    int64_t t_submit{}, t_launch{}, t_done{};
+    int status{SYCLFFT_HIP_STATUS_OK};
+    std::string err;
+    uint32_t group{};
+    bool done{false};
+};
+
+struct plan_entry
+{
+    std::string key;
+    void *plan{};
+    uint64_t capacity{}; // transforms one launch may hold (the plan's batch count)
+    uint64_t bytes{};    // device memory charged to the budget
+    int inflight{};      // launches in flight on this plan
+    int64_t last_use{};
+};
+
+struct launch_rec
+{
+    std::vector<std::shared_ptr<request>> reqs;
+    plan_entry *pe{};
+    hipEvent_t done{};
+    uint64_t bytes{};
+};
+
+static uint64_t elems_of(const syclfft_hip_request_v1 &r)
+{
+    uint64_t n = 1;
+    for (uint32_t i = 0; i < r.rank; ++i) {
+        n *= r.lengths[i];
+    }
+    return n;
+}
+
+class server
+{
+public:
+    server(int device, uint64_t budget, uint32_t nstreams, uint64_t max_batch)
+      : budget_(budget), max_batch_(max_batch ? max_batch : 512)
+    {
+        if (device < 0) {
+            if (hipGetDevice(&device) != hipSuccess) {
+                throw srv_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, "no current HIP device");
+            }
+        }
+        char why[512];
+        if (!syclfft_hip_supports(device, why, sizeof(why))) {
+            throw srv_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, why);
+        }
+        if (nstreams < 1 || nstreams > 16) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "streams must be 1..16");
+        }
+        if (budget == 0) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "memory budget must be positive");
+        }
+        device_ = device;
+        if (hipSetDevice(device_) != hipSuccess) {
+            throw srv_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, "hipSetDevice failed");
+        }
+        for (uint32_t i = 0; i < nstreams; ++i) {
+            hipStream_t s{};
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+                release_streams();
+                throw srv_error(SYCLFFT_HIP_STATUS_EXECUTION_FAILED, "hipStreamCreate failed");
+            }
+            streams_.push_back(s);
+        }
+        max_inflight_ = 2 * nstreams;
+        dispatcher_ = std::thread([this] { dispatch_loop(); });
+        completer_ = std::thread([this] { complete_loop(); });
+    }
+
+    ~server()
+    {
+        drain();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_dispatch_.notify_all();
+        cv_complete_.notify_all();
+        dispatcher_.join();
+        completer_.join();
+        (void)hipSetDevice(device_);
+        for (auto &pe : plans_) {
+            syclfft_hip_plan_destroy(pe->plan);
+        }
+        release_streams();
+    }
+
+    uint64_t submit(const syclfft_hip_request_v1 &r)
+    {
+        validate(r);
+        auto q = std::make_shared<request>();
+        q->r = r;
+        const uint64_t es = r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
+        const uint64_t data = elems_of(r) * r.batch_count * es;
+        check_buffer(r.in, data, "input");
+        if (!r.in_place) {
+            check_buffer(r.out, data, "output");
+        }
+        q->bytes = r.in_place ? data : 2 * data;
+        if (q->bytes > budget_) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "request footprint exceeds the memory budget");
+        }
+        char key[256];
+        std::snprintf(key, sizeof(key), "%u|%d|%u|%llux%llux%llu|%u|%u", r.scalar, r.direction, r.rank,
+                      static_cast<unsigned long long>(r.lengths[0]),
+                      static_cast<unsigned long long>(r.rank > 1 ? r.lengths[1] : 1),
+                      static_cast<unsigned long long>(r.rank > 2 ? r.lengths[2] : 1), r.normalization, r.in_place);
+        q->key = key;
+        q->t_submit = now_ns();
+        uint64_t t;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            t = next_ticket_++;
+            q->ticket = t;
+            pending_.push_back(q);
+            table_[t] = q;
+            ++outstanding_;
+            ++st_.submitted;
+        }
+        cv_dispatch_.notify_one();
+        return t;
+    }
+
+    int wait(uint64_t ticket, syclfft_hip_completion_v1 *out, std::string &err)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = table_.find(ticket);
+        if (it == table_.end()) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "unknown or already waited ticket");
+        }
+        std::shared_ptr<request> q = it->second;
+        cv_done_.wait(lk, [&] { return q->done; });
+        table_.erase(ticket);
+        if (out) {
+            out->ticket = q->ticket;
+            out->tag = q->r.tag;
+            out->status = q->status;
+            out->group_size = q->group;
+            const int64_t tl = q->t_launch ? q->t_launch : q->t_done;
+            out->queue_seconds = (tl - q->t_submit) * 1e-9;
+            out->run_seconds = (q->t_done - tl) * 1e-9;
+            out->latency_seconds = (q->t_done - q->t_submit) * 1e-9;
+            out->submit_at = (q->t_submit - t0_) * 1e-9;
+            out->launch_at = (tl - t0_) * 1e-9;
+            out->done_at = (q->t_done - t0_) * 1e-9;
+        }
+        err = q->err;
+        return q->status;
+    }
+
+    void drain()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return outstanding_ == 0; });
+    }
+
+    void stats(syclfft_hip_server_stats_v1 *s)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        *s = st_;
+        s->plans_resident = plans_.size();
+        s->budget_bytes = budget_;
+    }
+
+private:
+    void release_streams()
+    {
+        for (auto s : streams_) {
+            (void)hipStreamDestroy(s);
+        }
+        streams_.clear();
+    }
+
+    static void validate(const syclfft_hip_request_v1 &r)
+    {
+        if (r.abi_version != SYCLFFT_HIP_SERVER_ABI_VERSION) {
+            throw srv_error(SYCLFFT_HIP_STATUS_PLUGIN_ERROR, "request ABI version mismatch");
+        }
+        if (r.scalar != SYCLFFT_HIP_SCALAR_FLOAT && r.scalar != SYCLFFT_HIP_SCALAR_DOUBLE) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "scalar must be float or double");
+        }
+        if (r.direction != -1 && r.direction != 1) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "direction must be -1 or +1");
+        }
+        if (r.rank < 1 || r.rank > 3) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "rank must be 1..3");
+        }
+        for (uint32_t i = 0; i < r.rank; ++i) {
+            if (r.lengths[i] == 0) {
+                throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "lengths must be positive");
+            }
+        }
+        if (r.batch_count == 0) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "batch count must be positive");
+        }
+        if (r.normalization > SYCLFFT_HIP_NORM_ORTHOGONAL) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "unknown normalization");
+        }
+        if (!r.in || !r.out) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, "request buffers must be non-null");
+        }
+        if ((r.in == r.out) != (r.in_place != 0)) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                            r.in_place ? "in-place request needs in == out" : "out-of-place request needs in != out");
+        }
+    }
+
+    void check_buffer(const void *p, uint64_t bytes, const char *name) const
+    {
+        hipPointerAttribute_t attr{};
+        if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+            (void)hipGetLastError();
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " is not a HIP device allocation");
+        }
+        if (attr.type != hipMemoryTypeDevice || attr.device != device_) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " is not memory of the server's device");
+        }
+        hipDeviceptr_t base = nullptr;
+        size_t size = 0;
+        if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && base) {
+            const auto b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+            if (q < b || q - b + bytes > size) {
+                throw srv_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " allocation is smaller than the request");
+            }
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+
+    // Drop resident plans that no launch uses, least recently used first, until `need` more
+    // bytes fit the budget.  Returns whether they fit.
+    bool make_room(uint64_t need)
+    {
+        while (footprint_ + need > budget_) {
+            auto victim = plans_.end();
+            for (auto it = plans_.begin(); it != plans_.end(); ++it) {
+                if ((*it)->inflight == 0 && (victim == plans_.end() || (*it)->last_use < (*victim)->last_use)) {
+                    victim = it;
+                }
+            }
+            if (victim == plans_.end()) {
+                return false;
+            }
+            syclfft_hip_plan_destroy((*victim)->plan);
+            footprint_ -= (*victim)->bytes;
+            plans_.erase(victim);
+            ++st_.plans_evicted;
+        }
+        return true;
+    }
+
+    void fail(const std::vector<std::shared_ptr<request>> &g, int status, const std::string &msg)
+    {
+        const int64_t t = now_ns();
+        for (auto &q : g) {
+            q->status = status;
+            q->err = msg;
+            q->t_done = t;
+            q->group = static_cast<uint32_t>(g.size());
+            q->done = true;
+            --outstanding_;
+            ++st_.failed;
+        }
+        cv_done_.notify_all();
+    }
+
+    // Admit and launch the group of the oldest waiting request; false if it has to wait
+    // for a completion (budget or slots).
+    bool launch_next(std::unique_lock<std::mutex> &lk)
+    {
+        std::shared_ptr<request> head = pending_.front();
+        const uint64_t cap = std::max<uint64_t>(max_batch_, head->r.batch_count);
+        std::vector<std::shared_ptr<request>> group{head};
+        uint64_t units = head->r.batch_count, bytes = head->bytes;
+        size_t scanned = 0;
+        for (auto it = std::next(pending_.begin()); it != pending_.end() && scanned < 1024; ++it, ++scanned) {
+            const auto &q = *it;
+            if (q->key == head->key && units + q->r.batch_count <= cap && bytes + q->bytes <= budget_ / 2) {
+                group.push_back(q);
+                units += q->r.batch_count;
+                bytes += q->bytes;
+            }
+        }
+        // an idle resident plan of this shape large enough for the group; with one instance
+        // per stream already resident, the least busy of them (its executions serialize);
+        // else a new one
+        plan_entry *pe = nullptr, *busy = nullptr;
+        size_t instances = 0;
+        for (auto &e : plans_) {
+            if (e->key != head->key) {
+                continue;
+            }
+            ++instances;
+            if (e->capacity < units) {
+                continue;
+            }
+            if (e->inflight == 0 && (!pe || e->capacity < pe->capacity)) {
+                pe = e.get();
+            }
+            if (!busy || e->inflight < busy->inflight) {
+                busy = e.get();
+            }
+        }
+        if (!pe && busy && instances >= streams_.size()) {
+            pe = busy;
+        }
+        uint64_t capacity = 0, plan_bytes = 0;
+        if (!pe) {
+            capacity = 8;
+            while (capacity < units) {
+                capacity *= 2;
+            }
+            capacity = std::min(capacity, cap);
+            // charged up front: scratch of a multi-launch plan is about 2 transforms' worth per
+            // transform, tables about 4 transforms' worth once
+            const uint64_t es = head->r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
+            plan_bytes = (2 * capacity + 4) * elems_of(head->r) * es;
+        }
+        if (!make_room(bytes + plan_bytes)) {
+            if (!inflight_.empty()) {
+                return false; // wait for completions to free memory
+            }
+            // nothing in flight: the group can never fit as it is; shrink it to the oldest
+            // request, and fail that one if even it does not fit
+            if (group.size() > 1) {
+                group.resize(1);
+                units = head->r.batch_count;
+                bytes = head->bytes;
+                if (!pe) {
+                    capacity = 8;
+                    while (capacity < units) {
+                        capacity *= 2;
+                    }
+                    capacity = std::min(capacity, cap);
+                    const uint64_t es = head->r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
+                    plan_bytes = (2 * capacity + 4) * elems_of(head->r) * es;
+                }
+            }
+            if (group.size() > 1 || !make_room(bytes + plan_bytes)) {
+                pending_.pop_front();
+                fail(group, SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "request and its plan exceed the memory budget");
+                return true;
+            }
+        }
+        // reserve: take the group out of the queue, charge its footprint
+        for (auto &q : group) {
+            pending_.erase(std::find(pending_.begin(), pending_.end(), q));
+        }
+        footprint_ += bytes + plan_bytes;
+        st_.peak_footprint_bytes = std::max(st_.peak_footprint_bytes, footprint_);
+        if (pe) {
+            ++pe->inflight;
+        }
+        launch_rec rec;
+        rec.reqs = group;
+        rec.bytes = bytes;
+        hipStream_t stream = streams_[next_stream_++ % streams_.size()];
+        ++launching_;
+        lk.unlock();
+        // plan creation and the enqueue run without the lock (plan creation uploads tables)
+        std::string msg;
+        int status = SYCLFFT_HIP_STATUS_OK;
+        std::unique_ptr<plan_entry> fresh;
+        if (!pe) {
+            syclfft_hip_plan_config_v1 cfg{};
+            cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
+            cfg.scalar = head->r.scalar;
+            cfg.direction = head->r.direction;
+            cfg.rank = head->r.rank;
+            for (uint32_t i = 0; i < 3; ++i) {
+                cfg.lengths[i] = i < head->r.rank ? head->r.lengths[i] : 1;
+            }
+            cfg.batch_count = capacity;
+            cfg.in_place = head->r.in_place;
+            cfg.normalization = head->r.normalization;
+            cfg.device = device_;
+            char err[1024];
+            void *h = nullptr;
+            status = syclfft_hip_plan_create_ex(&cfg, &h, err, sizeof(err));
+            if (status == SYCLFFT_HIP_STATUS_OK) {
+                fresh = std::make_unique<plan_entry>();
+                fresh->key = head->key;
+                fresh->plan = h;
+                fresh->capacity = capacity;
+                fresh->bytes = plan_bytes;
+                fresh->inflight = 1;
+                pe = fresh.get();
+            } else {
+                msg = err;
+            }
+        }
+        if (status == SYCLFFT_HIP_STATUS_OK) {
+            std::vector<const void *> ins;
+            std::vector<void *> outs;
+            const uint64_t es = head->r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
+            const uint64_t tb = elems_of(head->r) * es;
+            for (auto &q : group) {
+                for (uint64_t b = 0; b < q->r.batch_count; ++b) {
+                    ins.push_back(static_cast<const unsigned char *>(q->r.in) + b * tb);
+                    outs.push_back(static_cast<unsigned char *>(q->r.out) + b * tb);
+                }
+            }
+            char err[1024];
+            if (hipEventCreateWithFlags(&rec.done, hipEventDisableTiming) != hipSuccess) {
+                status = SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+                msg = "hipEventCreate failed";
+            } else {
+                status = syclfft_hip_plan_execute_many(pe->plan, ins.size(), ins.data(), outs.data(), stream,
+                                                       nullptr, 0, rec.done, err, sizeof(err));
+                if (status != SYCLFFT_HIP_STATUS_OK) {
+                    msg = err;
+                    (void)hipEventDestroy(rec.done);
+                    rec.done = nullptr;
+                }
+            }
+        }
+        const int64_t t = now_ns();
+        lk.lock();
+        --launching_;
+        if (fresh) {
+            plans_.push_back(std::move(fresh));
+            ++st_.plans_created;
+        } else if (!pe) {
+            footprint_ -= plan_bytes; // plan creation failed
+        }
+        if (pe) {
+            pe->last_use = t;
+        }
+        if (status != SYCLFFT_HIP_STATUS_OK) {
+            footprint_ -= bytes;
+            if (pe) {
+                --pe->inflight;
+            }
+            fail(group, status, msg);
+            cv_dispatch_.notify_one();
+            return true;
+        }
+        for (auto &q : group) {
+            q->t_launch = t;
+            q->group = static_cast<uint32_t>(group.size());
+        }
+        rec.pe = pe;
+        inflight_.push_back(std::move(rec));
+        ++st_.launches;
+        cv_complete_.notify_one();
+        return true;
+    }
+
+    void dispatch_loop()
+    {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(mu_);
+        uint64_t seen = gen_;
+        for (;;) {
+            cv_dispatch_.wait(lk, [&] {
+                return (stop_ && pending_.empty()) || (!pending_.empty() && inflight_.size() < max_inflight_);
+            });
+            if (pending_.empty()) {
+                if (stop_) {
+                    return;
+                }
+                continue;
+            }
+            seen = gen_;
+            if (!launch_next(lk)) {
+                // wait until a completion changes the footprint
+                cv_dispatch_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            }
+        }
+    }
+
+    void complete_loop()
+    {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            if (inflight_.empty()) {
+                if (stop_ && pending_.empty() && launching_ == 0) {
+                    return;
+                }
+                cv_complete_.wait_for(lk, std::chrono::milliseconds(50));
+                continue;
+            }
+            bool any = false;
+            for (auto it = inflight_.begin(); it != inflight_.end();) {
+                const hipError_t q = hipEventQuery(it->done);
+                if (q == hipErrorNotReady) {
+                    ++it;
+                    continue;
+                }
+                const int64_t t = now_ns();
+                for (auto &r : it->reqs) {
+                    r->t_done = t;
+                    if (q != hipSuccess) {
+                        r->status = SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+                        r->err = hipGetErrorString(q);
+                        ++st_.failed;
+                    } else {
+                        ++st_.completed;
+                    }
+                    r->done = true;
+                    --outstanding_;
+                }
+                footprint_ -= it->bytes;
+                --it->pe->inflight;
+                it->pe->last_use = t;
+                (void)hipEventDestroy(it->done);
+                it = inflight_.erase(it);
+                any = true;
+            }
+            if (any) {
+                ++gen_;
+                cv_dispatch_.notify_one();
+                cv_done_.notify_all();
+            } else {
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            }
+        }
+    }
+
+    int device_{};
+    int64_t t0_{now_ns()};
+    uint64_t budget_, max_batch_;
+    size_t max_inflight_{};
+    std::vector<hipStream_t> streams_;
+    size_t next_stream_{0};
+    std::mutex mu_;
+    std::condition_variable cv_dispatch_, cv_done_, cv_complete_;
+    std::deque<std::shared_ptr<request>> pending_;
+    std::unordered_map<uint64_t, std::shared_ptr<request>> table_;
+    std::list<std::unique_ptr<plan_entry>> plans_;
+    std::list<launch_rec> inflight_;
+    uint64_t footprint_{0};
+    uint64_t next_ticket_{1};
+    uint64_t outstanding_{0};
+    uint64_t gen_{0};
+    int launching_{0};
+    bool stop_{false};
+    syclfft_hip_server_stats_v1 st_{};
+    std::thread dispatcher_, completer_;
+};
+
+} // namespace sfh_srv
+
+using sfh_srv::srv_error;
+
+extern "C" void *syclfft_hip_server_create(
+    int device, uint64_t memory_budget_bytes, uint32_t streams, uint64_t max_batch_transforms, char *err, size_t cap)
+{
+    try {
+        sfh_srv::set_err(err, cap, "");
+        return new sfh_srv::server(device, memory_budget_bytes, streams, max_batch_transforms);
+    } catch (const std::exception &ex) {
+        sfh_srv::set_err(err, cap, ex.what());
+    } catch (...) {
+        sfh_srv::set_err(err, cap, "unknown error");
+    }
+    return nullptr;
+}
+
+extern "C" int syclfft_hip_server_submit(
+    void *server, const syclfft_hip_request_v1 *request, uint64_t *ticket, char *err, size_t cap)
+{
+    try {
+        sfh_srv::set_err(err, cap, "");
+        if (!server || !request || !ticket) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "null server, request or ticket");
+        }
+        *ticket = static_cast<sfh_srv::server *>(server)->submit(*request);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const srv_error &ex) {
+        sfh_srv::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh_srv::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+extern "C" int syclfft_hip_server_wait(
+    void *server, uint64_t ticket, syclfft_hip_completion_v1 *out, char *err, size_t cap)
+{
+    try {
+        sfh_srv::set_err(err, cap, "");
+        if (!server) {
+            throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "null server");
+        }
+        std::string msg;
+        const int st = static_cast<sfh_srv::server *>(server)->wait(ticket, out, msg);
+        sfh_srv::set_err(err, cap, msg);
+        return st;
+    } catch (const srv_error &ex) {
+        sfh_srv::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh_srv::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+extern "C" int syclfft_hip_server_drain(void *server, char *err, size_t cap)
+{
+    sfh_srv::set_err(err, cap, "");
+    if (!server) {
+        sfh_srv::set_err(err, cap, "null server");
+        return SYCLFFT_HIP_STATUS_INVALID_ARGUMENT;
+    }
+    static_cast<sfh_srv::server *>(server)->drain();
+    return SYCLFFT_HIP_STATUS_OK;
+}
+
+extern "C" void syclfft_hip_server_stats(void *server, syclfft_hip_server_stats_v1 *out)
+{
+    if (server && out) {
+        static_cast<sfh_srv::server *>(server)->stats(out);
+    }
+}
+
+extern "C" void syclfft_hip_server_destroy(void *server)
+{
+    delete static_cast<sfh_srv::server *>(server);
+}

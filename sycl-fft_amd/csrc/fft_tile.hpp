@@ -944,12 +944,28 @@ __device__ __forceinline__ void set_tile(const sfh_tile_args &a, tile_ctx<T> &c,
     c.blo = static_cast<uint32_t>(blo);
     const uint64_t left = a.wcount - w0;
     c.wv = static_cast<uint32_t>(left < a.W ? left : a.W);
-    c.tsrc = static_cast<const cx<T> *>(a.in)
-             + (static_cast<int64_t>(bhi) * a.in_bhi + static_cast<int64_t>(blo) * a.in_blo
-                + static_cast<int64_t>(w0) * a.in_w);
-    c.tdst = static_cast<cx<T> *>(a.out)
-             + (static_cast<int64_t>(bhi) * a.out_bhi + static_cast<int64_t>(blo) * a.out_blo
-                + static_cast<int64_t>(w0) * a.out_w);
+    // Unit tables replace in/out + (index along unit_dim) * stride by a per-unit base.  The
+    // BLO/BHI cases are uniform over the workgroup; the COL case is per column and is
+    // resolved per lane below (such launches never use the LDS load/store loops).
+    const uint32_t ud = a.unit_dim;
+    if (a.in_units && ud != SFH_UNIT_COL) {
+        const cx<T> *u = static_cast<const cx<T> *>(a.in_units[ud == SFH_UNIT_BLO ? blo : bhi]);
+        c.tsrc = u + (ud == SFH_UNIT_BLO ? static_cast<int64_t>(bhi) * a.in_bhi : static_cast<int64_t>(blo) * a.in_blo)
+                 + static_cast<int64_t>(w0) * a.in_w;
+    } else {
+        c.tsrc = static_cast<const cx<T> *>(a.in)
+                 + (static_cast<int64_t>(bhi) * a.in_bhi + static_cast<int64_t>(blo) * a.in_blo
+                    + static_cast<int64_t>(w0) * a.in_w);
+    }
+    if (a.out_units && ud != SFH_UNIT_COL) {
+        cx<T> *u = static_cast<cx<T> *>(a.out_units[ud == SFH_UNIT_BLO ? blo : bhi]);
+        c.tdst = u + (ud == SFH_UNIT_BLO ? static_cast<int64_t>(bhi) * a.out_bhi : static_cast<int64_t>(blo) * a.out_blo)
+                 + static_cast<int64_t>(w0) * a.out_w;
+    } else {
+        c.tdst = static_cast<cx<T> *>(a.out)
+                 + (static_cast<int64_t>(bhi) * a.out_bhi + static_cast<int64_t>(blo) * a.out_blo
+                    + static_cast<int64_t>(w0) * a.out_w);
+    }
     if (a.W == 1) {
         c.w = 0; // NT need not be a power of two here
         c.lt = tid;
@@ -963,6 +979,15 @@ __device__ __forceinline__ void set_tile(const sfh_tile_args &a, tile_ctx<T> &c,
     c.wvalid = c.w < c.wv;
     c.src = c.tsrc + static_cast<int32_t>(c.w) * a.in_w;
     c.dst = c.tdst + static_cast<int32_t>(c.w) * a.out_w;
+    if (ud == SFH_UNIT_COL) {
+        const uint32_t col = c.wvalid ? c.w0 + c.w : c.w0; // invalid lanes never access memory
+        if (a.in_units) {
+            c.src = static_cast<const cx<T> *>(a.in_units[col]);
+        }
+        if (a.out_units) {
+            c.dst = static_cast<cx<T> *>(a.out_units[col]);
+        }
+    }
     c.col = (MODE != 0) ? col_of(a, c.w0, c.blo, c.w) : 0u;
 }
 

@@ -48,8 +48,15 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_plan_destroy",
     "syclfft_hip_plan_scratch_bytes",
     "syclfft_hip_plan_execute",
+    "syclfft_hip_plan_execute_many",
     "syclfft_hip_plan_describe",
     "syclfft_hip_plan_launch_count",
+    "syclfft_hip_server_create",
+    "syclfft_hip_server_submit",
+    "syclfft_hip_server_wait",
+    "syclfft_hip_server_drain",
+    "syclfft_hip_server_stats",
+    "syclfft_hip_server_destroy",
 )
 
 
@@ -129,15 +136,162 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
         ctypes.c_size_t]
     lib.syclfft_hip_plan_execute.restype = ctypes.c_int
+    lib.syclfft_hip_plan_execute_many.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
+        ctypes.c_size_t]
+    lib.syclfft_hip_plan_execute_many.restype = ctypes.c_int
     lib.syclfft_hip_plan_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.syclfft_hip_plan_describe.restype = ctypes.c_size_t
     lib.syclfft_hip_plan_launch_count.argtypes = [ctypes.c_void_p]
     lib.syclfft_hip_plan_launch_count.restype = ctypes.c_uint32
+    lib.syclfft_hip_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                              ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_server_create.restype = ctypes.c_void_p
+    lib.syclfft_hip_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_server_submit.restype = ctypes.c_int
+    lib.syclfft_hip_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Completion),
+                                            ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_server_wait.restype = ctypes.c_int
+    lib.syclfft_hip_server_drain.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_server_drain.restype = ctypes.c_int
+    lib.syclfft_hip_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+    lib.syclfft_hip_server_stats.restype = None
+    lib.syclfft_hip_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.syclfft_hip_server_destroy.restype = None
     lib.syclfft_get_device_provider_v1.argtypes = []
     lib.syclfft_get_device_provider_v1.restype = ctypes.POINTER(ProviderTable)
     if path is None:
         _lib = lib
     return lib
+
+
+SERVER_ABI_VERSION = 1
+
+
+class Request(ctypes.Structure):
+    """syclfft_hip_request_v1 (include/syclfft_hip_server.h)."""
+    _fields_ = [
+        ("abi_version", ctypes.c_uint32),
+        ("scalar", ctypes.c_uint32),
+        ("direction", ctypes.c_int32),
+        ("rank", ctypes.c_uint32),
+        ("lengths", ctypes.c_uint64 * 3),
+        ("batch_count", ctypes.c_uint64),
+        ("normalization", ctypes.c_uint32),
+        ("in_place", ctypes.c_uint32),
+        ("inp", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("tag", ctypes.c_uint64),
+    ]
+
+
+class Completion(ctypes.Structure):
+    """syclfft_hip_completion_v1."""
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("tag", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("group_size", ctypes.c_uint32),
+        ("queue_seconds", ctypes.c_double),
+        ("run_seconds", ctypes.c_double),
+        ("latency_seconds", ctypes.c_double),
+        ("submit_at", ctypes.c_double),
+        ("launch_at", ctypes.c_double),
+        ("done_at", ctypes.c_double),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    """syclfft_hip_server_stats_v1."""
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "completed", "failed", "launches", "plans_created", "plans_resident", "plans_evicted",
+        "peak_footprint_bytes", "budget_bytes")]
+
+
+class Server:
+    """The provider's serving loop: continuous batching of transform requests over
+    resident plans (include/syclfft_hip_server.h)."""
+
+    def __init__(self, budget_bytes: int, streams: int = 4, max_batch: int = 512, device: int = -1):
+        self._lib = load()
+        err = ctypes.create_string_buffer(1024)
+        h = self._lib.syclfft_hip_server_create(int(device), int(budget_bytes), int(streams), int(max_batch),
+                                                err, 1024)
+        if not h:
+            raise SyclfftError("provider_unavailable", err.value.decode())
+        self._h = ctypes.c_void_p(h)
+
+    def submit_ptr(self, in_ptr: int, out_ptr: int, lengths: Sequence[int], batch: int = 1,
+                   direction: int = FORWARD, double: bool = False, normalization: str = "none",
+                   tag: int = 0) -> int:
+        r = Request()
+        r.abi_version = SERVER_ABI_VERSION
+        r.scalar = SCALAR_DOUBLE if double else SCALAR_FLOAT
+        r.direction = int(direction)
+        r.rank = len(lengths)
+        for i, n in enumerate(list(lengths)[:3]):
+            r.lengths[i] = int(n)
+        r.batch_count = int(batch)
+        r.normalization = NORM[normalization] if isinstance(normalization, str) else int(normalization)
+        r.in_place = 1 if in_ptr == out_ptr else 0
+        r.inp = in_ptr
+        r.out = out_ptr
+        r.tag = int(tag)
+        t = ctypes.c_uint64()
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_server_submit(self._h, ctypes.byref(r), ctypes.byref(t), err, 1024)
+        if st != 0:
+            _raise(st, err)
+        return int(t.value)
+
+    def submit(self, x, y=None, lengths=None, batch: int = 1, direction: int = FORWARD,
+               normalization: str = "none", tag: int = 0) -> int:
+        """Queue a transform of torch tensor x into y (y=None: in place)."""
+        import torch
+
+        y = x if y is None else y
+        return self.submit_ptr(x.data_ptr(), y.data_ptr(), lengths, batch, direction,
+                               x.dtype == torch.complex128, normalization, tag)
+
+    def wait(self, ticket: int) -> dict:
+        c = Completion()
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_server_wait(self._h, int(ticket), ctypes.byref(c), err, 1024)
+        if st != 0:
+            _raise(st, err)
+        return {"ticket": c.ticket, "tag": c.tag, "group_size": c.group_size, "queue_s": c.queue_seconds,
+                "run_s": c.run_seconds, "latency_s": c.latency_seconds, "submit_at": c.submit_at,
+                "launch_at": c.launch_at, "done_at": c.done_at}
+
+    def drain(self) -> None:
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_server_drain(self._h, err, 1024)
+        if st != 0:
+            _raise(st, err)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self._lib.syclfft_hip_server_stats(self._h, ctypes.byref(s))
+        return {n: int(getattr(s, n)) for n, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            self._lib.syclfft_hip_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def provider_table(lib: Optional[ctypes.CDLL] = None) -> ProviderTable:
@@ -251,6 +405,35 @@ class Plan:
             ctypes.c_void_p(stream), arr, len(dl), ctypes.c_void_p(done), err, 1024)
         if st != 0:
             _raise(st, err)
+
+    def execute_many_ptr(self, ins: Sequence[int], outs: Sequence[int], stream: int = 0,
+                         deps: Iterable[int] = (), done: int = 0) -> None:
+        """Enqueue len(ins) transforms of the plan's shape over raw device pointers
+        (transform k reads ins[k], writes outs[k]); at most the plan's batch count."""
+        if len(ins) != len(outs):
+            raise ValueError("ins and outs must have the same length")
+        n = len(ins)
+        ia = (ctypes.c_void_p * max(1, n))(*[int(v) for v in ins])
+        oa = (ctypes.c_void_p * max(1, n))(*[int(v) for v in outs])
+        dl = [int(d) for d in deps]
+        arr = (ctypes.c_void_p * max(1, len(dl)))(*dl) if dl else None
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_plan_execute_many(
+            self.handle, n, ia, oa, ctypes.c_void_p(stream), arr, len(dl), ctypes.c_void_p(done), err, 1024)
+        if st != 0:
+            _raise(st, err)
+
+    def execute_many(self, xs, ys=None, stream=None, deps=(), done=None) -> None:
+        """Execute on lists of torch tensors, one transform of the plan's shape each."""
+        import torch
+
+        ys = xs if ys is None else ys
+        dev = xs[0].device
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        self.execute_many_ptr([x.data_ptr() for x in xs], [y.data_ptr() for y in ys], s.cuda_stream,
+                              [e.cuda_event for e in deps], 0)
+        if done is not None:
+            done.record(s)
 
     def execute(self, x, y=None, stream=None, deps=(), done=None) -> None:
         """Execute on torch tensors (complex64 / complex128 on a HIP device).

@@ -2,7 +2,9 @@
 // (TEST INFRASTRUCTURE ONLY).  Usage: emu_driver <case>...  with a case written as
 //   f|d:n0[xn1[xn2]]:batch:dir:norm:inplace      e.g.  f:4096:2:-1:0:0
 // Prints one line per case: "ok <case> relL2=<e>" or "FAIL <case> ..."; exit status 1 on
-// any failure.
+// any failure.  EMU_MANY=1 executes through syclfft_hip_plan_execute_many with every
+// transform in its own allocation (allocated in reverse order); EMU_MANY=2 does the same for
+// the first half of the batch only (a count below the plan's batch count).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -81,11 +83,47 @@ static bool run_case(const std::string &spec)
             x[i] = f[i];
         }
     }
-    const int est = syclfft_hip_plan_execute(plan, din, dout, nullptr, nullptr, 0, nullptr, err, sizeof(err));
-    if (est != 0) {
-        std::printf("FAIL %s execute status %d: %s\n", spec.c_str(), est, err);
-        return false;
+    const char *many_env = std::getenv("EMU_MANY");
+    const int many = many_env ? std::atoi(many_env) : 0;
+    uint64_t tsz = 1;
+    for (auto n : lens) {
+        tsz *= n;
     }
+    uint64_t count = batch;
+    if (many) {
+        count = many == 2 ? (batch + 1) / 2 : batch;
+        std::vector<void *> ins(count), outs(count);
+        for (uint64_t k = count; k-- > 0;) {
+            hipMalloc(&ins[k], tsz * es);
+            std::memcpy(ins[k], static_cast<unsigned char *>(din) + k * tsz * es, tsz * es);
+            if (inplace) {
+                outs[k] = ins[k];
+            } else {
+                hipMalloc(&outs[k], tsz * es);
+            }
+        }
+        const int est = syclfft_hip_plan_execute_many(plan, count, ins.data(), outs.data(), nullptr, nullptr, 0,
+                                                      nullptr, err, sizeof(err));
+        if (est != 0) {
+            std::printf("FAIL %s execute_many status %d: %s\n", spec.c_str(), est, err);
+            return false;
+        }
+        for (uint64_t k = 0; k < count; ++k) {
+            std::memcpy(static_cast<unsigned char *>(dout) + k * tsz * es, outs[k], tsz * es);
+            if (!inplace) {
+                hipFree(outs[k]);
+            }
+            hipFree(ins[k]);
+        }
+    } else {
+        const int est =
+            syclfft_hip_plan_execute(plan, din, dout, nullptr, nullptr, 0, nullptr, err, sizeof(err));
+        if (est != 0) {
+            std::printf("FAIL %s execute status %d: %s\n", spec.c_str(), est, err);
+            return false;
+        }
+    }
+    total = count * tsz; // compare the transforms that ran
     if (dbl) {
         std::memcpy(got.data(), dout, total * 16);
     } else {
@@ -94,7 +132,7 @@ static bool run_case(const std::string &spec)
             got[i] = f[i];
         }
     }
-    oracle_fftn_ld(static_cast<int>(lens.size()), lens.data(), batch, x.data(), ref.data(), dir, norm);
+    oracle_fftn_ld(static_cast<int>(lens.size()), lens.data(), count, x.data(), ref.data(), dir, norm);
     double num = 0, den = 0;
     for (uint64_t i = 0; i < 2 * total; ++i) {
         num += (got[i] - ref[i]) * (got[i] - ref[i]);

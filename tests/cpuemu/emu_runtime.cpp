@@ -129,6 +129,21 @@ hipError_t hipFree(void *p)
     std::free(p);
     return hipSuccess;
 }
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t)
+{
+    std::memcpy(d, s, n); // work runs in issue order, synchronously
+    return hipSuccess;
+}
+hipError_t hipHostMalloc(void **p, size_t n, unsigned)
+{
+    *p = std::malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void *p)
+{
+    std::free(p);
+    return hipSuccess;
+}
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
 {
     std::memcpy(d, s, n);
@@ -226,6 +241,11 @@ hipError_t hipEventQuery(hipEvent_t)
     return hipSuccess;
 }
 hipError_t hipStreamCreate(hipStream_t *s)
+{
+    *s = nullptr;
+    return hipSuccess;
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
 {
     *s = nullptr;
     return hipSuccess;

@@ -14,7 +14,8 @@ enum
     hipErrorInvalidValue = 1,
     hipErrorOutOfMemory = 2,
     hipErrorInvalidConfiguration = 9,
-    hipErrorInvalidDevice = 101
+    hipErrorInvalidDevice = 101,
+    hipErrorNotReady = 600
 };
 typedef struct ihipStream_t *hipStream_t;
 typedef struct ihipEvent_t *hipEvent_t;
@@ -70,6 +71,10 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *, int);
 hipError_t hipMalloc(void **, size_t);
 hipError_t hipFree(void *);
 hipError_t hipMemcpy(void *, const void *, size_t, hipMemcpyKind);
+hipError_t hipMemcpyAsync(void *, const void *, size_t, hipMemcpyKind, hipStream_t);
+#define hipHostMallocDefault 0u
+hipError_t hipHostMalloc(void **, size_t, unsigned);
+hipError_t hipHostFree(void *);
 hipError_t hipMemsetAsync(void *, int, size_t, hipStream_t);
 hipError_t hipMemset(void *, int, size_t);
 hipError_t hipEventCreate(hipEvent_t *);
@@ -94,4 +99,6 @@ hipError_t hipDeviceGetAttribute(int *, hipDeviceAttribute_t, int);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *, const void *, int, size_t);
 hipError_t hipEventQuery(hipEvent_t);
 hipError_t hipStreamCreate(hipStream_t *);
+#define hipStreamNonBlocking 1u
+hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);

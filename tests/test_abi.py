@@ -3,6 +3,7 @@ validates plan configurations the way the reference does before any device work
 (checked_element_count, src/core/provider_loader.cpp:112-132; direction check,
 src/sycl/sycl.cpp:435-438).  No compute call is made here."""
 import ctypes
+import glob
 import os
 import re
 
@@ -12,12 +13,15 @@ from conftest import ROOT
 
 import syclfft_hip as sh
 
-HEADER = os.path.join(ROOT, "include", "syclfft_hip_provider.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared_symbols():
-    text = "\n".join(l for l in open(HEADER).read().splitlines() if not l.lstrip().startswith("#"))
-    return sorted(set(re.findall(r"SYCLFFT_HIP_API[^;(]*?\b(\w+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = "\n".join(l for l in open(h).read().splitlines() if not l.lstrip().startswith("#"))
+        names.update(re.findall(r"SYCLFFT_HIP_API[^;(]*?\b(\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_symbols_match_binding_list():
@@ -106,4 +110,22 @@ def test_no_gpu_reports_unavailable():
     assert not ok and reason
     with pytest.raises(sh.SyclfftError) as e:
         sh.Plan([8])
+    assert e.value.code == "provider_unavailable"
+
+
+def test_server_struct_layout():
+    # syclfft_hip_request_v1: 4 x u32, 3 x u64 lengths, u64 batch, 2 x u32, 2 pointers, u64 tag
+    assert ctypes.sizeof(sh.Request) == 16 + 24 + 8 + 8 + 16 + 8
+    assert sh.Request.lengths.offset == 16 and sh.Request.inp.offset == 56 and sh.Request.tag.offset == 72
+    assert ctypes.sizeof(sh.Completion) == 8 + 8 + 4 + 4 + 6 * 8
+    assert ctypes.sizeof(sh.ServerStats) == 9 * 8
+
+
+def test_server_without_gpu_reports_unavailable():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(sh.SyclfftError) as e:
+        sh.Server(1 << 30)
     assert e.value.code == "provider_unavailable"

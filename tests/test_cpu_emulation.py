@@ -78,3 +78,36 @@ def test_emulated_measure_mode_writes_cache(tmp_path):
     assert "f32" in text and "4096x1x1|9" in text
     key, value = text.strip().split("\t")
     assert value.startswith("s") and "w" in value
+
+
+MANY = ["f:4096:3:-1:0:0", "f:300:5:1:3:0", "d:2520:3:1:3:1", "f:10007:3:-1:3:0", "f:32768:3:-1:0:0",
+        "d:1009:3:-1:3:1", "f:13x256:3:-1:0:0", "d:256x13:3:1:0:0", "f:32x32x32:3:1:0:1",
+        "d:64x48x10:3:-1:1:0", "f:16384:1:1:1:1"]
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_emulated_pointer_array_execution(mode):
+    """syclfft_hip_plan_execute_many: every transform in its own allocation (mode 1), and
+    a count below the plan's batch count (mode 2), across single, four-step, fused and
+    three-launch Bluestein and multi-D plans."""
+    _run(MANY, {"EMU_MANY": mode})
+    _run(["f:10007:3:-1:3:0", "d:1009:5:1:2:1", "f:4096:3:1:0:0", "d:1031x3:3:-1:0:0"],
+         {"EMU_MANY": mode, "SYCLFFT_HIP_TUNE": "s1w0p0g0"})
+
+
+EMU_SERVER = os.path.join(os.path.dirname(EMU), "emu_server")
+
+
+def test_emulated_serving_loop():
+    """The serving loop on the emulated device: a mix of 36 requests (6 shapes, both
+    types, directions, normalizations, in-place and out-of-place) submitted at once is
+    merged into shared launches, each result matches the FP64 oracle and the same request
+    run alone; budget, ticket and host-buffer errors are rejected."""
+    if not os.path.exists(EMU_SERVER):
+        pytest.fail(f"{EMU_SERVER} is not built (run make -C tests/cpuemu)")
+    env = dict(os.environ)
+    env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=0"
+    r = subprocess.run([EMU_SERVER], capture_output=True, text=True, timeout=900, env=env)
+    lines = r.stdout.splitlines()
+    bad = [l for l in lines if not l.startswith("ok ")]
+    assert r.returncode == 0 and not bad and len(lines) > 100, (r.stdout[-4000:], r.stderr[-4000:])
