@@ -16,9 +16,11 @@ Also reported: the dominant kernel's roofline (HIP events around back-to-back la
 it on its stream), an fp32 batched sweep N = 2^12..2^20 at 512 MiB per buffer (HBM roofline
 fraction per N), rel-L2 of each sub-workload against the FP64 oracle on sampled lines, and
 the reference-algorithm CPU baseline (the oracle's restatement of the portable provider,
-OpenMP on the host cores) on a bounded sample.
+OpenMP on the host cores) on a bounded sample, and the serving loop (batched capacity,
+isolated latency, p50/p99 latency at 80% of capacity) on a seeded mixed request stream.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep] [--no-overlap]
+                       [--no-serve] [--serve-requests R]
 """
 import argparse
 import ctypes
@@ -98,6 +100,122 @@ def cpu_baseline(lib, sample_scale=16, target_s=12.0):
     return total_flops / elapsed / 1e9, reps, sample
 
 
+def smooth7(n):
+    for f in (2, 3, 5, 7):
+        while n % f == 0:
+            n //= f
+    return n == 1
+
+
+def is_prime(n):
+    if n < 2:
+        return False
+    f = 2
+    while f * f <= n:
+        if n % f == 0:
+            return False
+        f += 1
+    return True
+
+
+def serving_requests(nreq, seed):
+    """A seeded request stream of the serving configuration: lengths 50% powers of two
+    2^8..2^20, 30% 7-smooth 240..500000, 20% primes 257..65537 (Bluestein), one in ten
+    requests 2-D; batch count log-uniform 1..512, capped at 2^22 points per request;
+    80% complex<float>, 20% complex<double>; random direction and normalization."""
+    g = np.random.default_rng(seed)
+    pow2 = [1 << k for k in range(8, 21)]
+    sm = [n for n in range(240, 500001) if smooth7(n)]
+    smooth = sorted({sm[int(i)] for i in np.linspace(0, len(sm) - 1, 24)})
+    pr = [n for n in range(257, 65538) if is_prime(n)]
+    primes = sorted({pr[int(i)] for i in np.geomspace(1, len(pr), 12) - 1})
+    two_d = [(64, 64), (128, 128), (256, 256), (512, 512), (1024, 1024), (120, 240), (250, 250)]
+    norms = ["none", "forward", "backward", "orthogonal"]
+    reqs = []
+    for _ in range(nreq):
+        u = g.random()
+        if g.random() < 0.1:
+            lengths = list(two_d[int(g.integers(len(two_d)))])
+        elif u < 0.5:
+            lengths = [pow2[int(g.integers(len(pow2)))]]
+        elif u < 0.8:
+            lengths = [smooth[int(g.integers(len(smooth)))]]
+        else:
+            lengths = [primes[int(g.integers(len(primes)))]]
+        n = int(np.prod(lengths))
+        batch = int(np.exp(g.uniform(0, np.log(512))))
+        batch = max(1, min(batch, (1 << 22) // n))
+        reqs.append(dict(lengths=lengths, n=n, batch=batch, double=bool(g.random() < 0.2),
+                         direction=int(g.choice([-1, 1])), norm=norms[int(g.integers(4))]))
+    return reqs
+
+
+def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, streams=8):
+    """The serving loop under three loads over the same request stream: all requests at
+    once (batched capacity), one at a time (isolated latency), and Poisson arrivals at 80%
+    of the batched request rate (latency under load).  Request buffers are device-resident:
+    inputs read from one seeded buffer, each request's output in its own region of one
+    arena.  Arrivals are replayed from native threads (syclfft_hip_server_replay)."""
+    reqs = serving_requests(nreq, seed)
+    flops_total = sum(flops(r["n"], r["batch"]) for r in reqs)
+    src = torch.empty((1 << 22) * 2, dtype=torch.float64, device=dev).uniform_(-1, 1)
+    src64 = torch.view_as_complex(src.view(-1, 2))
+    src32 = torch.view_as_complex(src[: (1 << 23)].float().view(-1, 2))
+    sizes = [r["n"] * r["batch"] * (16 if r["double"] else 8) for r in reqs]
+    offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in sizes])])
+    arena = torch.empty(int(offs[-1]), dtype=torch.uint8, device=dev)
+    base = arena.data_ptr()
+    torch.cuda.synchronize()
+    srv = sh.Server(budget_bytes=budget_gb << 30, streams=streams, max_batch=512)
+    requests = [sh.Server.make_request((src64 if r["double"] else src32).data_ptr(), base + int(offs[k]),
+                                       r["lengths"], r["batch"], r["direction"], r["double"], r["norm"], tag=k)
+                for k, r in enumerate(reqs)]
+
+    def span(recs):
+        return max(c["done_at"] for c in recs) - min(c["submit_at"] for c in recs)
+
+    srv.replay(requests)  # warm-up: plans for every shape become resident
+    batched = srv.replay(requests)
+    span_b = span(batched)
+    rate = len(reqs) / span_b
+    isolated = srv.replay(requests, sequential=True)
+    g = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(g.exponential(1.0 / (0.8 * rate), len(reqs)))
+    loaded = srv.replay(requests, arrivals=arrivals)
+    span_l = span(loaded)
+    st = srv.stats()
+    srv.close()
+    del arena, src, src64, src32
+    lat_l = np.array([c["latency_s"] for c in loaded]) * 1e3
+    lat_i = np.array([c["latency_s"] for c in isolated]) * 1e3
+    slow = lat_l / np.maximum(lat_i, 1e-6)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 4)  # noqa: E731
+    return {
+        "requests": len(reqs),
+        "sample": f"{len(reqs)} requests, seed {seed}: lengths 50% pow2 2^8..2^20, 30% 7-smooth 240..500000 "
+                  f"(24 sizes), 20% primes 257..65537 (12 sizes), 10% 2-D; batch log-uniform 1..512 capped at "
+                  f"2^22 points; 80% f32 / 20% f64; random direction and normalization; budget {budget_gb} GiB; "
+                  f"{streams} streams; arrivals replayed natively",
+        "batched_gflops": round(flops_total / span_b / 1e9, 2),
+        "batched_request_rate": round(rate, 1),
+        "offered_request_rate": round(0.8 * rate, 1),
+        "loaded_gflops": round(flops_total / span_l / 1e9, 2),
+        "loaded_frac_of_batched": round(span_b / span_l, 4),
+        "p50_ms": pct(lat_l, 50),
+        "p99_ms": pct(lat_l, 99),
+        "isolated_p50_ms": pct(lat_i, 50),
+        "isolated_p99_ms": pct(lat_i, 99),
+        "p99_over_isolated_p99": round(float(np.percentile(lat_l, 99) / np.percentile(lat_i, 99)), 3),
+        "per_request_slowdown_p50": pct(slow, 50),
+        "per_request_slowdown_p99": pct(slow, 99),
+        "mean_group_size_batched": round(float(np.mean([c["group_size"] for c in batched])), 2),
+        "mean_group_size_loaded": round(float(np.mean([c["group_size"] for c in loaded])), 2),
+        "launches": st["launches"],
+        "plans_resident": st["plans_resident"],
+        "peak_footprint_gib": round(st["peak_footprint_bytes"] / (1 << 30), 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +224,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-overlap", action="store_true", help="skip the three-stream report")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every transform of this workload fits one MI355X and "
@@ -287,6 +407,12 @@ def main():
             p.close()
         del xs, ys
 
+    # ---------------- serving loop (continuous batching over resident plans)
+    serving = None
+    if not args.no_serve:
+        serving = serving_bench(sh, torch, dev, nreq=args.serve_requests)
+        log(f"serving: {serving}")
+
     # ---------------- CPU baseline (reference algorithm, restated; host cores)
     cpu = None
     if not args.no_cpu:
@@ -327,6 +453,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
+        "serving": serving,
     }
     print(json.dumps(line))
 

@@ -90,6 +90,16 @@ SYCLFFT_HIP_API int syclfft_hip_server_drain(void *server, char *err, size_t cap
 
 SYCLFFT_HIP_API void syclfft_hip_server_stats(void *server, syclfft_hip_server_stats_v1 *out);
 
+/* Load replay for measurements: submit requests[k] when arrival_seconds[k] has elapsed since
+ * the call started (busy-waiting on a steady clock; arrival_seconds = NULL submits them back
+ * to back), or, with sequential != 0, each only after the previous one has completed; wait
+ * for all of them and fill completions[k].  Submission and waiting run in native threads,
+ * so the arrival process is not limited by the caller's language runtime.  Returns OK, or
+ * the first failing request's status (its message in err). */
+SYCLFFT_HIP_API int syclfft_hip_server_replay(
+    void *server, const syclfft_hip_request_v1 *requests, const double *arrival_seconds, size_t count,
+    int sequential, syclfft_hip_completion_v1 *completions, char *err, size_t cap);
+
 /* Drain, then release streams and resident plans. */
 SYCLFFT_HIP_API void syclfft_hip_server_destroy(void *server);
 

@@ -1668,8 +1668,13 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
 // Device-pointer check: the whole [ptr, ptr+bytes) range must lie in one device
 // allocation visible to the plan's device (replaces the USM check of validate_pointer,
 // src/sycl/sycl.cpp:72-85).  A host pointer would fault the GPU, so it is rejected here.
-static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, const char *name)
+// Returns the allocation range [*lo, *hi) when the runtime reports it (0, 0 otherwise).
+static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, const char *name,
+                             uintptr_t *lo = nullptr, uintptr_t *hi = nullptr)
 {
+    if (lo) {
+        *lo = *hi = 0;
+    }
     if (!ptr) {
         throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, std::string(name) + " must be non-null");
     }
@@ -1697,6 +1702,10 @@ static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, con
             throw sfh_error(
                 SYCLFFT_HIP_STATUS_INVALID_POINTER,
                 std::string(name) + " allocation is smaller than the transform");
+        }
+        if (lo) {
+            *lo = b;
+            *hi = b + size;
         }
     } else {
         (void)hipGetLastError();
@@ -1762,9 +1771,12 @@ static void execute_plan(
 // touches a caller buffer gets a unit table (one base pointer per unit of its transform
 // dimension) in a device slot of the plan's ring; every launch's transform dimension is cut
 // to `count` transforms, so scratch holds the first `count` transforms' intermediates.
+// `validate` = false: the caller checked the buffers as whole allocations; `ordered` = false:
+// the caller orders executions itself, so a plan without scratch (whose executions share no
+// memory) does not wait for its previous execution.
 static void execute_units(
     plan &p, uint64_t count, const void *const *ins, void *const *outs, hipStream_t stream, void *const *deps,
-    size_t ndeps, hipEvent_t done)
+    size_t ndeps, hipEvent_t done, bool validate = true, bool ordered = true)
 {
     if (p.copy_only || p.launches.empty()) {
         throw sfh_error(
@@ -1778,9 +1790,21 @@ static void execute_units(
         throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, "pointer arrays must be non-null");
     }
     const uint64_t bytes = p.transform * p.esize;
+    // every pointer is checked; pointers inside an allocation already checked (the
+    // transforms of one contiguous batch) only need the range test
+    uintptr_t lo[2] = {0, 0}, hi[2] = {0, 0};
+    const auto check = [&](const void *ptr, int side, const char *name) {
+        const auto q = reinterpret_cast<uintptr_t>(ptr);
+        if (hi[side] && q >= lo[side] && q + bytes <= hi[side]) {
+            return;
+        }
+        validate_pointer(p, ptr, bytes, name, &lo[side], &hi[side]);
+    };
     for (uint64_t k = 0; k < count; ++k) {
-        validate_pointer(p, ins[k], bytes, "input");
-        validate_pointer(p, outs[k], bytes, "output");
+        if (validate) {
+            check(ins[k], 0, "input");
+            check(outs[k], 1, "output");
+        }
         if (p.cfg.in_place != (ins[k] == outs[k] ? 1u : 0u)) {
             throw sfh_error(
                 SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
@@ -1793,6 +1817,14 @@ static void execute_units(
                 throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
             }
         }
+    }
+    // Transforms laid out like a contiguous batch (ins[k] = ins[0] + k transform sizes, same
+    // for outs) need no tables: the launches run on ins[0] / outs[0] with their transform
+    // dimension cut to `count`.
+    bool affine = true;
+    for (uint64_t k = 1; k < count && affine; ++k) {
+        affine = static_cast<const unsigned char *>(ins[k]) == static_cast<const unsigned char *>(ins[0]) + k * bytes
+                 && static_cast<const unsigned char *>(outs[k]) == static_cast<const unsigned char *>(outs[0]) + k * bytes;
     }
     // table layout: per launch, an input table then an output table where the side is a
     // caller buffer; sized for the plan's batch count
@@ -1812,7 +1844,10 @@ static void execute_units(
     }
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
-    if (!p.unit_bytes) {
+    if (affine) {
+        cap_entries = 0;
+    }
+    if (!affine && !p.unit_bytes) {
         p.unit_bytes = std::max<uint64_t>(cap_entries, 1) * sizeof(void *);
         for (int k = 0; k < plan::k_unit_slots; ++k) {
             hip_check(hipMalloc(&p.unit_dev[k], p.unit_bytes), "hipMalloc(unit tables)");
@@ -1820,13 +1855,16 @@ static void execute_units(
             hip_check(hipEventCreateWithFlags(&p.unit_free[k], hipEventDisableTiming), "hipEventCreate");
         }
     }
-    const int slot = p.unit_next;
-    p.unit_next = (p.unit_next + 1) % plan::k_unit_slots;
-    if (p.unit_used[slot]) {
-        hip_check(hipEventSynchronize(p.unit_free[slot]), "hipEventSynchronize(unit slot)");
+    const int slot = affine ? -1 : p.unit_next;
+    void **host = nullptr, **dev = nullptr;
+    if (!affine) {
+        p.unit_next = (p.unit_next + 1) % plan::k_unit_slots;
+        if (p.unit_used[slot]) {
+            hip_check(hipEventSynchronize(p.unit_free[slot]), "hipEventSynchronize(unit slot)");
+        }
+        host = static_cast<void **>(p.unit_host[slot]);
+        dev = static_cast<void **>(p.unit_dev[slot]);
     }
-    void **host = static_cast<void **>(p.unit_host[slot]);
-    void **dev = static_cast<void **>(p.unit_dev[slot]);
     struct side_tables
     {
         const void *const *in;
@@ -1848,7 +1886,7 @@ static void execute_units(
         pos += count * units;
         return d;
     };
-    for (size_t i = 0; i < p.launches.size(); ++i) {
+    for (size_t i = 0; i < p.launches.size() && !affine; ++i) {
         const launch &l = p.launches[i];
         if (l.src != BUF_SCRATCH) {
             tabs[i].in = fill(l, l.src, l.unit_in_stride);
@@ -1862,7 +1900,8 @@ static void execute_units(
             hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
         }
     }
-    if (p.have_last) {
+    const bool serialize = ordered || p.scratch != nullptr;
+    if (serialize && p.have_last) {
         hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
     }
     if (pos) {
@@ -1874,8 +1913,9 @@ static void execute_units(
         sfh_tile_args a = l.args;
         const uint64_t units = l.unit_extent / B;
         const uint64_t ext = count * units;
-        a.in = (l.src == BUF_SCRATCH) ? p.scratch : nullptr;
-        a.out = (l.dst == BUF_SCRATCH) ? p.scratch : nullptr;
+        const void *user[2] = {ins[0], outs[0]}; // BUF_IN, BUF_OUT bases of the affine case
+        a.in = (l.src == BUF_SCRATCH) ? p.scratch : (affine ? user[l.src] : nullptr);
+        a.out = (l.dst == BUF_SCRATCH) ? p.scratch : (affine ? const_cast<void *>(user[l.dst]) : nullptr);
         a.in_units = tabs[i].in;
         a.out_units = tabs[i].out;
         const uint64_t nbhi = a.ntiles / (a.nwt * a.nblo);
@@ -1896,10 +1936,14 @@ static void execute_units(
                 &p.variants[l.variant], l.mode, &a, static_cast<uint32_t>(std::min<uint64_t>(grid, 0xffffffffu)), stream)),
             "tile kernel launch");
     }
-    hip_check(hipEventRecord(p.unit_free[slot], stream), "hipEventRecord(unit slot)");
-    p.unit_used[slot] = true;
-    hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
-    p.have_last = true;
+    if (!affine) {
+        hip_check(hipEventRecord(p.unit_free[slot], stream), "hipEventRecord(unit slot)");
+        p.unit_used[slot] = true;
+    }
+    if (serialize) {
+        hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+        p.have_last = true;
+    }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
     }
@@ -2055,6 +2099,29 @@ extern "C" int syclfft_hip_plan_execute_many(
         return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
     } catch (...) {
         sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+// Same as syclfft_hip_plan_execute_many for callers inside the library whose buffers were
+// validated as whole allocations already (the serving loop checks each request's buffers
+// when it is submitted) and which order executions themselves (each request has its own
+// buffers); only the per-transform placement rules are still checked, and executions of a
+// plan without scratch may overlap.
+extern "C" __attribute__((visibility("hidden"))) int sfh_plan_execute_many_checked_buffers(
+    void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *done, char *err,
+    size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        sfh::execute_units(*static_cast<sfh::plan *>(plan), count, ins, outs, static_cast<hipStream_t>(stream),
+                           nullptr, 0, static_cast<hipEvent_t>(done), false, false);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
         return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
     }
 }

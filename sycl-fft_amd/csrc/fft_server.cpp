@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -34,6 +35,11 @@
 #include <vector>
 
 #include "syclfft_hip_server.h"
+
+// fft_plan.cpp: pointer-array execution for buffers validated as whole allocations
+extern "C" int sfh_plan_execute_many_checked_buffers(
+    void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *done, char *err,
+    size_t cap);
 
 namespace sfh_srv
 {
@@ -90,6 +96,7 @@ struct launch_rec
     plan_entry *pe{};
     hipEvent_t done{};
     uint64_t bytes{};
+    size_t stream{};
 };
 
 static uint64_t elems_of(const syclfft_hip_request_v1 &r)
@@ -134,7 +141,8 @@ public:
             }
             streams_.push_back(s);
         }
-        max_inflight_ = 2 * nstreams;
+        max_inflight_ = 4 * nstreams;
+        stream_load_.assign(nstreams, 0);
         dispatcher_ = std::thread([this] { dispatch_loop(); });
         completer_ = std::thread([this] { complete_loop(); });
     }
@@ -153,6 +161,9 @@ public:
         (void)hipSetDevice(device_);
         for (auto &pe : plans_) {
             syclfft_hip_plan_destroy(pe->plan);
+        }
+        for (auto e : events_) {
+            (void)hipEventDestroy(e);
         }
         release_streams();
     }
@@ -235,6 +246,27 @@ public:
     }
 
 private:
+    // completion events are recycled (creating one per launch costs more than the launch)
+    hipEvent_t take_event()
+    {
+        {
+            std::lock_guard<std::mutex> lk(ev_mu_);
+            if (!events_.empty()) {
+                hipEvent_t e = events_.back();
+                events_.pop_back();
+                return e;
+            }
+        }
+        hipEvent_t e = nullptr;
+        return hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess ? e : nullptr;
+    }
+
+    void give_event(hipEvent_t e)
+    {
+        std::lock_guard<std::mutex> lk(ev_mu_);
+        events_.push_back(e);
+    }
+
     void release_streams()
     {
         for (auto s : streams_) {
@@ -315,6 +347,8 @@ private:
             }
             syclfft_hip_plan_destroy((*victim)->plan);
             footprint_ -= (*victim)->bytes;
+            auto &same = by_key_[(*victim)->key];
+            same.erase(std::find(same.begin(), same.end(), victim->get()));
             plans_.erase(victim);
             ++st_.plans_evicted;
         }
@@ -358,19 +392,17 @@ private:
         // else a new one
         plan_entry *pe = nullptr, *busy = nullptr;
         size_t instances = 0;
-        for (auto &e : plans_) {
-            if (e->key != head->key) {
-                continue;
-            }
+        auto &same = by_key_[head->key];
+        for (plan_entry *e : same) {
             ++instances;
             if (e->capacity < units) {
                 continue;
             }
             if (e->inflight == 0 && (!pe || e->capacity < pe->capacity)) {
-                pe = e.get();
+                pe = e;
             }
             if (!busy || e->inflight < busy->inflight) {
-                busy = e.get();
+                busy = e;
             }
         }
         if (!pe && busy && instances >= streams_.size()) {
@@ -426,7 +458,19 @@ private:
         launch_rec rec;
         rec.reqs = group;
         rec.bytes = bytes;
-        hipStream_t stream = streams_[next_stream_++ % streams_.size()];
+        // the stream with the least work in flight (bytes of the launches queued on it; round
+        // robin among equals), so a short request does not queue behind a long one when
+        // another stream is free
+        size_t si = next_stream_++ % streams_.size();
+        for (size_t k = 0; k < streams_.size(); ++k) {
+            const size_t c = (si + k) % streams_.size();
+            if (stream_load_[c] < stream_load_[si]) {
+                si = c;
+            }
+        }
+        stream_load_[si] += bytes;
+        rec.stream = si;
+        hipStream_t stream = streams_[si];
         ++launching_;
         lk.unlock();
         // plan creation and the enqueue run without the lock (plan creation uploads tables)
@@ -473,15 +517,16 @@ private:
                 }
             }
             char err[1024];
-            if (hipEventCreateWithFlags(&rec.done, hipEventDisableTiming) != hipSuccess) {
+            rec.done = take_event();
+            if (!rec.done) {
                 status = SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
                 msg = "hipEventCreate failed";
             } else {
-                status = syclfft_hip_plan_execute_many(pe->plan, ins.size(), ins.data(), outs.data(), stream,
-                                                       nullptr, 0, rec.done, err, sizeof(err));
+                status = sfh_plan_execute_many_checked_buffers(pe->plan, ins.size(), ins.data(), outs.data(), stream,
+                                                               rec.done, err, sizeof(err));
                 if (status != SYCLFFT_HIP_STATUS_OK) {
                     msg = err;
-                    (void)hipEventDestroy(rec.done);
+                    give_event(rec.done);
                     rec.done = nullptr;
                 }
             }
@@ -490,6 +535,7 @@ private:
         lk.lock();
         --launching_;
         if (fresh) {
+            by_key_[fresh->key].push_back(fresh.get());
             plans_.push_back(std::move(fresh));
             ++st_.plans_created;
         } else if (!pe) {
@@ -500,6 +546,7 @@ private:
         }
         if (status != SYCLFFT_HIP_STATUS_OK) {
             footprint_ -= bytes;
+            stream_load_[si] -= bytes;
             if (pe) {
                 --pe->inflight;
             }
@@ -523,10 +570,18 @@ private:
         (void)hipSetDevice(device_);
         std::unique_lock<std::mutex> lk(mu_);
         uint64_t seen = gen_;
+        const auto ready = [&] {
+            return (stop_ && pending_.empty()) || (!pending_.empty() && inflight_.size() < max_inflight_);
+        };
         for (;;) {
-            cv_dispatch_.wait(lk, [&] {
-                return (stop_ && pending_.empty()) || (!pending_.empty() && inflight_.size() < max_inflight_);
-            });
+            // stay awake briefly after work (a condition-variable wake-up costs tens of
+            // microseconds of request latency), then sleep
+            for (int spin = 0; spin < 2000 && !ready(); ++spin) {
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            }
+            cv_dispatch_.wait(lk, ready);
             if (pending_.empty()) {
                 if (stop_) {
                     return;
@@ -553,13 +608,25 @@ private:
                 cv_complete_.wait_for(lk, std::chrono::milliseconds(50));
                 continue;
             }
-            bool any = false;
-            for (auto it = inflight_.begin(); it != inflight_.end();) {
+            // query the launches' events without holding the lock (the dispatcher only
+            // appends to inflight_ and only this thread erases, so iterators stay valid)
+            std::vector<std::pair<std::list<launch_rec>::iterator, hipError_t>> ready;
+            std::vector<std::list<launch_rec>::iterator> snap;
+            for (auto it = inflight_.begin(); it != inflight_.end(); ++it) {
+                snap.push_back(it);
+            }
+            lk.unlock();
+            for (auto it : snap) {
                 const hipError_t q = hipEventQuery(it->done);
-                if (q == hipErrorNotReady) {
-                    ++it;
-                    continue;
+                if (q != hipErrorNotReady) {
+                    ready.emplace_back(it, q);
                 }
+            }
+            lk.lock();
+            bool any = false;
+            for (auto &rq : ready) {
+                auto it = rq.first;
+                const hipError_t q = rq.second;
                 const int64_t t = now_ns();
                 for (auto &r : it->reqs) {
                     r->t_done = t;
@@ -574,10 +641,11 @@ private:
                     --outstanding_;
                 }
                 footprint_ -= it->bytes;
+                stream_load_[it->stream] -= it->bytes;
                 --it->pe->inflight;
                 it->pe->last_use = t;
-                (void)hipEventDestroy(it->done);
-                it = inflight_.erase(it);
+                give_event(it->done);
+                inflight_.erase(it);
                 any = true;
             }
             if (any) {
@@ -597,12 +665,14 @@ private:
     uint64_t budget_, max_batch_;
     size_t max_inflight_{};
     std::vector<hipStream_t> streams_;
+    std::vector<uint64_t> stream_load_; // bytes of the launches in flight per stream
     size_t next_stream_{0};
     std::mutex mu_;
     std::condition_variable cv_dispatch_, cv_done_, cv_complete_;
     std::deque<std::shared_ptr<request>> pending_;
     std::unordered_map<uint64_t, std::shared_ptr<request>> table_;
     std::list<std::unique_ptr<plan_entry>> plans_;
+    std::unordered_map<std::string, std::vector<plan_entry *>> by_key_; // resident plans per shape
     std::list<launch_rec> inflight_;
     uint64_t footprint_{0};
     uint64_t next_ticket_{1};
@@ -611,6 +681,8 @@ private:
     int launching_{0};
     bool stop_{false};
     syclfft_hip_server_stats_v1 st_{};
+    std::mutex ev_mu_;
+    std::vector<hipEvent_t> events_;
     std::thread dispatcher_, completer_;
 };
 
@@ -688,6 +760,85 @@ extern "C" void syclfft_hip_server_stats(void *server, syclfft_hip_server_stats_
     if (server && out) {
         static_cast<sfh_srv::server *>(server)->stats(out);
     }
+}
+
+extern "C" int syclfft_hip_server_replay(
+    void *server, const syclfft_hip_request_v1 *requests, const double *arrival_seconds, size_t count,
+    int sequential, syclfft_hip_completion_v1 *completions, char *err, size_t cap)
+{
+    sfh_srv::set_err(err, cap, "");
+    if (!server || (count && (!requests || !completions))) {
+        sfh_srv::set_err(err, cap, "null server, requests or completions");
+        return SYCLFFT_HIP_STATUS_INVALID_ARGUMENT;
+    }
+    auto *srv = static_cast<sfh_srv::server *>(server);
+    std::vector<uint64_t> tickets(count, 0);
+    std::vector<int> status(count, SYCLFFT_HIP_STATUS_OK);
+    std::vector<std::string> msgs(count);
+    std::atomic<size_t> submitted{0};
+    const auto submit_one = [&](size_t k) {
+        try {
+            tickets[k] = srv->submit(requests[k]);
+        } catch (const srv_error &ex) {
+            status[k] = ex.status;
+            msgs[k] = ex.what();
+        } catch (const std::exception &ex) {
+            status[k] = SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+            msgs[k] = ex.what();
+        }
+    };
+    const auto wait_one = [&](size_t k) {
+        completions[k] = syclfft_hip_completion_v1{};
+        completions[k].tag = requests[k].tag;
+        if (status[k] != SYCLFFT_HIP_STATUS_OK) {
+            completions[k].status = status[k];
+            return;
+        }
+        try {
+            std::string m;
+            status[k] = srv->wait(tickets[k], &completions[k], m);
+            if (status[k] != SYCLFFT_HIP_STATUS_OK) {
+                msgs[k] = m;
+            }
+        } catch (const srv_error &ex) {
+            status[k] = ex.status;
+            msgs[k] = ex.what();
+        }
+    };
+    if (sequential) {
+        for (size_t k = 0; k < count; ++k) {
+            submit_one(k);
+            wait_one(k);
+        }
+    } else {
+        std::thread waiter([&] {
+            for (size_t k = 0; k < count; ++k) {
+                while (submitted.load(std::memory_order_acquire) <= k) {
+                    std::this_thread::yield();
+                }
+                wait_one(k);
+            }
+        });
+        const auto t0 = std::chrono::steady_clock::now();
+        for (size_t k = 0; k < count; ++k) {
+            if (arrival_seconds) {
+                const auto due = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                          std::chrono::duration<double>(arrival_seconds[k]));
+                while (std::chrono::steady_clock::now() < due) {
+                }
+            }
+            submit_one(k);
+            submitted.store(k + 1, std::memory_order_release);
+        }
+        waiter.join();
+    }
+    for (size_t k = 0; k < count; ++k) {
+        if (status[k] != SYCLFFT_HIP_STATUS_OK) {
+            sfh_srv::set_err(err, cap, msgs[k]);
+            return status[k];
+        }
+    }
+    return SYCLFFT_HIP_STATUS_OK;
 }
 
 extern "C" void syclfft_hip_server_destroy(void *server)

@@ -56,6 +56,7 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_server_wait",
     "syclfft_hip_server_drain",
     "syclfft_hip_server_stats",
+    "syclfft_hip_server_replay",
     "syclfft_hip_server_destroy",
 )
 
@@ -158,6 +159,10 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.syclfft_hip_server_drain.restype = ctypes.c_int
     lib.syclfft_hip_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
     lib.syclfft_hip_server_stats.restype = None
+    lib.syclfft_hip_server_replay.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.c_int,
+                                              ctypes.POINTER(Completion), ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_server_replay.restype = ctypes.c_int
     lib.syclfft_hip_server_destroy.argtypes = [ctypes.c_void_p]
     lib.syclfft_hip_server_destroy.restype = None
     lib.syclfft_get_device_provider_v1.argtypes = []
@@ -270,6 +275,40 @@ class Server:
         st = self._lib.syclfft_hip_server_drain(self._h, err, 1024)
         if st != 0:
             _raise(st, err)
+
+    @staticmethod
+    def make_request(in_ptr: int, out_ptr: int, lengths: Sequence[int], batch: int = 1,
+                     direction: int = FORWARD, double: bool = False, normalization: str = "none",
+                     tag: int = 0) -> "Request":
+        r = Request()
+        r.abi_version = SERVER_ABI_VERSION
+        r.scalar = SCALAR_DOUBLE if double else SCALAR_FLOAT
+        r.direction = int(direction)
+        r.rank = len(lengths)
+        for i, n in enumerate(list(lengths)[:3]):
+            r.lengths[i] = int(n)
+        r.batch_count = int(batch)
+        r.normalization = NORM[normalization] if isinstance(normalization, str) else int(normalization)
+        r.in_place = 1 if in_ptr == out_ptr else 0
+        r.inp = in_ptr
+        r.out = out_ptr
+        r.tag = int(tag)
+        return r
+
+    def replay(self, requests: Sequence["Request"], arrivals=None, sequential: bool = False) -> list:
+        """Submit the requests (at `arrivals` seconds, back to back, or one at a time) from
+        native threads and return their completion records in order."""
+        n = len(requests)
+        arr = (Request * max(1, n))(*requests)
+        comp = (Completion * max(1, n))()
+        times = (ctypes.c_double * n)(*[float(t) for t in arrivals]) if arrivals is not None else None
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_server_replay(self._h, arr, times, n, 1 if sequential else 0, comp, err, 1024)
+        if st != 0:
+            _raise(st, err)
+        return [{"ticket": c.ticket, "tag": c.tag, "group_size": c.group_size, "queue_s": c.queue_seconds,
+                 "run_s": c.run_seconds, "latency_s": c.latency_seconds, "submit_at": c.submit_at,
+                 "launch_at": c.launch_at, "done_at": c.done_at} for c in comp[:n]]
 
     def stats(self) -> dict:
         s = ServerStats()

@@ -4,7 +4,8 @@
 // Prints one line per case: "ok <case> relL2=<e>" or "FAIL <case> ..."; exit status 1 on
 // any failure.  EMU_MANY=1 executes through syclfft_hip_plan_execute_many with every
 // transform in its own allocation (allocated in reverse order); EMU_MANY=2 does the same for
-// the first half of the batch only (a count below the plan's batch count).
+// the first half of the batch only (a count below the plan's batch count); EMU_MANY=3 passes
+// pointers into the contiguous batch for its first half (no unit tables).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -90,7 +91,22 @@ static bool run_case(const std::string &spec)
         tsz *= n;
     }
     uint64_t count = batch;
-    if (many) {
+    if (many == 3) {
+        // pointers into the contiguous batch (the affine case: no unit tables), first half
+        count = (batch + 1) / 2;
+        std::vector<const void *> ins(count);
+        std::vector<void *> outs(count);
+        for (uint64_t k = 0; k < count; ++k) {
+            ins[k] = static_cast<unsigned char *>(din) + k * tsz * es;
+            outs[k] = static_cast<unsigned char *>(dout) + k * tsz * es;
+        }
+        const int est = syclfft_hip_plan_execute_many(plan, count, ins.data(), outs.data(), nullptr, nullptr, 0,
+                                                      nullptr, err, sizeof(err));
+        if (est != 0) {
+            std::printf("FAIL %s execute_many status %d: %s\n", spec.c_str(), est, err);
+            return false;
+        }
+    } else if (many) {
         count = many == 2 ? (batch + 1) / 2 : batch;
         std::vector<void *> ins(count), outs(count);
         for (uint64_t k = count; k-- > 0;) {

@@ -85,11 +85,12 @@ MANY = ["f:4096:3:-1:0:0", "f:300:5:1:3:0", "d:2520:3:1:3:1", "f:10007:3:-1:3:0"
         "d:64x48x10:3:-1:1:0", "f:16384:1:1:1:1"]
 
 
-@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("mode", ["1", "2", "3"])
 def test_emulated_pointer_array_execution(mode):
-    """syclfft_hip_plan_execute_many: every transform in its own allocation (mode 1), and
-    a count below the plan's batch count (mode 2), across single, four-step, fused and
-    three-launch Bluestein and multi-D plans."""
+    """syclfft_hip_plan_execute_many: every transform in its own allocation (mode 1), a
+    count below the plan's batch count (mode 2), and pointers into one contiguous batch
+    (mode 3, no unit tables), across single, four-step, fused and three-launch Bluestein
+    and multi-D plans."""
     _run(MANY, {"EMU_MANY": mode})
     _run(["f:10007:3:-1:3:0", "d:1009:5:1:2:1", "f:4096:3:1:0:0", "d:1031x3:3:-1:0:0"],
          {"EMU_MANY": mode, "SYCLFFT_HIP_TUNE": "s1w0p0g0"})

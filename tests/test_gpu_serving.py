@@ -141,10 +141,10 @@ def test_serving_continuous_arrivals():
 def test_serving_rejects_bad_requests():
     say("case serving errors")
     with sh.Server(budget_bytes=1 << 20, streams=1) as srv:
-        x = torch.zeros(1 << 16, dtype=torch.complex64, device="cuda")
+        x = torch.zeros(1 << 17, dtype=torch.complex64, device="cuda")
         y = torch.zeros_like(x)
         with pytest.raises(sh.SyclfftError) as e:
-            srv.submit(x, y, [1 << 16], 1)  # 1 MiB in + 1 MiB out > 1 MiB budget
+            srv.submit(x, y, [1 << 17], 1)  # 1 MiB in + 1 MiB out > 1 MiB budget
         assert e.value.code == "invalid_argument"
         host = np.zeros(16, dtype=np.complex64)
         with pytest.raises(sh.SyclfftError) as e:
