@@ -216,6 +216,36 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
     }
 
 
+def multid_bench(sh, torch, dev, reps=10):
+    """The multi-D configuration: 3-D 256^3 complex<float> and 2-D 4096^2 complex<double>,
+    in place, forward normalization (autotuned plans); per-execute time, GB/s and GFLOP/s."""
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    for name, shape, dbl in (("c3_f32_256x256x256", [256, 256, 256], False),
+                             ("c3_f64_4096x4096", [4096, 4096], True)):
+        n = int(np.prod(shape))
+        es = 16 if dbl else 8
+        x = torch.empty(n, dtype=torch.complex128 if dbl else torch.complex64, device=dev)
+        x.real.uniform_(-1, 1)
+        x.imag.uniform_(-1, 1)
+        p = sh.Plan(shape, 1, sh.FORWARD, double=dbl, in_place=True, normalization="forward", measure=True)
+        for _ in range(2):
+            p.execute(x)
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            p.execute(x)
+        b_.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b_) / reps
+        out[name] = {"ms": round(ms, 4), "gbs": round(2 * n * es / (ms * 1e-3) / 1e9, 1),
+                     "gflops": round(flops(n, 1) / (ms * 1e-3) / 1e9, 1), "launches": p.launch_count(),
+                     "plan": " | ".join(l for l in p.describe().strip().split("\n"))}
+        p.close()
+        del x
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +437,9 @@ def main():
             p.close()
         del xs, ys
 
+    # ---------------- multi-D configuration
+    multid = None if args.no_sweep else multid_bench(sh, torch, dev)
+
     # ---------------- serving loop (continuous batching over resident plans)
     serving = None
     if not args.no_serve:
@@ -453,6 +486,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
+        "multid": multid,
         "serving": serving,
     }
     print(json.dumps(line))
