@@ -870,7 +870,17 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
                 g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
             }
             uint64_t s1 = 0, s2 = 0;
-            const bool can_split = p.tune.split && split_two(p, n, 1, s1, s2);
+            bool can_split = p.tune.split && split_two(p, n, 1, s1, s2);
+            if (inner > 1 && !can_split && n <= static_cast<uint64_t>(max_cap(p))) {
+                // A strided axis whose tile holds fewer than min_cols columns at full length
+                // reads row segments of under 128 B; the four-step form reads whole 16-64
+                // column rows in both of its passes, which beats the one short-segment pass.
+                uint32_t wfit = g.W;
+                while (wfit > 1 && !fits_single(p, n, wfit)) {
+                    wfit /= 2;
+                }
+                can_split = wfit < min_cols(p) && split_two(p, n, min_cols(p), s1, s2);
+            }
             if (!can_split && n <= static_cast<uint64_t>(max_cap(p)) && fits_single(p, n, g.W)) {
                 p.launches.push_back(make_launch(p, SFH_MI_PLAIN, g, src, dst, false, "single"));
                 return;
