@@ -8,6 +8,7 @@ SFH_DECLARE(f32_64_16)
 SFH_DECLARE(f32_128_16)
 SFH_DECLARE(f32_256_16)
 SFH_DECLARE(f32_512_16)
+SFH_DECLARE(f32_512_20)
 SFH_DECLARE(f32_1024_16)
 SFH_DECLARE(f32_1024_20)
 SFH_DECLARE(f64_64_16)
@@ -29,7 +30,8 @@ struct variant_spec
 const variant_spec k_f32[] = {
     {sfh_kernels_f32_64_16, 64, 16},    {sfh_kernels_f32_128_16, 128, 16},
     {sfh_kernels_f32_256_16, 256, 16},  {sfh_kernels_f32_512_16, 512, 16},
-    {sfh_kernels_f32_1024_16, 1024, 16}, {sfh_kernels_f32_1024_20, 1024, 20},
+    {sfh_kernels_f32_512_20, 512, 20},  {sfh_kernels_f32_1024_16, 1024, 16},
+    {sfh_kernels_f32_1024_20, 1024, 20},
 };
 const variant_spec k_f64[] = {
     {sfh_kernels_f64_64_16, 64, 16},   {sfh_kernels_f64_128_16, 128, 16},

@@ -962,11 +962,127 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
         }
 }
 
+// Bluestein with M = 2 Mh >= 2n over two workgroups per transform, for batches too small to
+// fill the device with one workgroup per transform.  Since n <= Mh, the zero-padded
+// a = x c has an all-zero upper half, so the even and odd halves of DFT_M(a) are the
+// length-Mh DFTs of a and of a W_M^j (j < Mh).  Half h runs the convolution for its half of
+// the spectrum, X_h[k] = DFT_M(a)[2k + h], in one workgroup:
+//   E_h = IDFT_Mh(X_h * S[2k + h])  with S = DFT_M(b) / M,
+// and the inverse splits the same way, z[m] = E_0[m] + W_M^-m E_1[m] for m < n.  Launch 1
+// writes P_h[m] = c[m] E_0[m] and c[m] W_M^-m E_1[m] (m < n) to scratch; launch 2, a radix-2
+// tile of the pair whose difference output is skipped (POST bound 1), writes
+// y[m] = P_0[m] + P_1[m].
+static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id src, buf_id dst)
+{
+    const uint64_t cap = static_cast<uint64_t>(max_cap(p));
+    const uint64_t Mh = smooth_at_least(n, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+    if (!Mh) {
+        return false;
+    }
+    const uint64_t M = 2 * Mh;
+    std::vector<cxl> chirp(n), bvec(M, cxl{0.0L, 0.0L});
+    for (uint64_t j = 0; j < n; ++j) {
+        chirp[j] = chirp_ld(j, n);
+        const cxl b{chirp[j].re, -chirp[j].im};
+        bvec[j] = b;
+        if (j) {
+            bvec[M - j] = b;
+        }
+    }
+    std::vector<cxl> spec = host_dft_ld(bvec);
+    const long double invM = 1.0L / static_cast<long double>(M);
+    for (auto &v : spec) {
+        v.re *= invM;
+        v.im *= invM;
+    }
+    const auto mul = [](cxl a, cxl b) { return cxl{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; };
+    // Interleaved by half so that one bound gives both the zero padding and the truncation:
+    // pre[2j + h] = c[j] W_M^(h j) is read iff 2j + h < 2n, i.e. j < n;
+    // post[2m + h] = conj(c[m] W_M^(-h m)) (conj-folded) is written iff m < n.
+    std::vector<cxl> pre(2 * n), post(2 * n);
+    for (uint64_t j = 0; j < n; ++j) {
+        pre[2 * j] = chirp[j];
+        pre[2 * j + 1] = mul(chirp[j], root_ld(j, M));
+        post[2 * j] = cxl{chirp[j].re, -chirp[j].im};
+        const cxl c1 = mul(chirp[j], root_ld(M - j, M)); // c[m] W_M^-m
+        post[2 * j + 1] = cxl{c1.re, -c1.im};
+    }
+    const std::string nkey = std::to_string(n) + "_" + std::to_string(M);
+    const int64_t pre_off = table(p, "hpre" + nkey, pre);
+    const int64_t post_off = table(p, "hpost" + nkey, post);
+    const int64_t spec_off = table(p, "spec" + nkey, spec);
+    const int64_t one_off = table(p, "one", std::vector<cxl>{cxl{1.0L, 0.0L}});
+    p.scratch_elems = std::max(p.scratch_elems, outer * 2 * n);
+    {
+        geometry g;
+        g.L = static_cast<uint32_t>(Mh);
+        g.W = 1;
+        g.wcount = outer;
+        g.unit_dim = SFH_UNIT_COL;
+        g.nblo = 2; // the half h
+        g.in_w = static_cast<int64_t>(n);
+        g.in_j = 1;
+        g.in_blo = 0; // both halves read the same transform
+        g.out_w = static_cast<int64_t>(2 * n);
+        g.out_k = 1;
+        g.out_blo = static_cast<int64_t>(n);
+        g.col_div = 1;
+        g.col_mod = 1;
+        g.col_bmul = 1; // col = h
+        launch l = make_launch(p, SFH_MI_BLUE_FUSED, g, src, BUF_SCRATCH, true, "bluestein-half");
+        sfh_tile_args &a = l.args;
+        a.pre_jmul = 2;
+        a.pre_cmul = 1;
+        a.nvalid_in = static_cast<uint32_t>(2 * n);
+        a.mid_kmul = 2;
+        a.mid_cmul = 1;
+        a.post_kmul = 2;
+        a.post_cmul = 1;
+        a.nvalid_out = static_cast<uint32_t>(2 * n);
+        l.conj_post = true;
+        l.pre_off = pre_off;
+        l.mid_off = spec_off;
+        l.post_off = post_off;
+        p.launches.push_back(l);
+    }
+    {
+        geometry g;
+        g.L = 2;
+        g.W = choose_w(p, 2, n, true);
+        g.wcount = n;
+        g.nblo = outer;
+        g.unit_dim = SFH_UNIT_BLO;
+        g.in_w = 1;
+        g.in_j = static_cast<int64_t>(n);
+        g.in_blo = static_cast<int64_t>(2 * n);
+        g.out_w = 1;
+        g.out_k = static_cast<int64_t>(n);
+        g.out_blo = static_cast<int64_t>(n);
+        launch l = make_launch(p, SFH_MI_BLUE_C, g, BUF_SCRATCH, dst, false, "bluestein-sum");
+        l.args.post_kmul = 1; // q = k: only the sum output k = 0 of the pair is written
+        l.args.post_cmul = 0;
+        l.args.nvalid_out = 1;
+        l.post_off = one_off;
+        p.launches.push_back(l);
+    }
+    return true;
+}
+
 static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_id src, buf_id dst)
 {
     // ---------------- Bluestein ----------------
     // chirp c[j] = exp(-i pi j^2/n); b[m] = conj(c[m]) for |m| < n wrapped mod M;
     // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
+    if (inner == 1 && !p.tune.split) {
+        // Two workgroups per transform, on request (SYCLFFT_HIP_BLUE_HALVES=1).  Measured on
+        // n = 10007: one workgroup's program of two FFTs is latency-bound, so halving its
+        // length only shortens it from ~55 to ~46 us, and the pair-sum launch costs more
+        // than that (batch 128: 63 us against 55 us fused); off by default.
+        const char *e = std::getenv("SYCLFFT_HIP_BLUE_HALVES");
+        if (e && std::atoi(e) != 0 && plan_bluestein_halves(p, outer, n, src, dst)) {
+            return;
+        }
+    }
     const uint64_t need = 2 * n - 1;
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
     const uint64_t fusedM = smooth_at_least(need, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });

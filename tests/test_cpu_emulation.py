@@ -112,3 +112,16 @@ def test_emulated_serving_loop():
     lines = r.stdout.splitlines()
     bad = [l for l in lines if not l.startswith("ok ")]
     assert r.returncode == 0 and not bad and len(lines) > 100, (r.stdout[-4000:], r.stderr[-4000:])
+
+
+HALVES = ["f:10007:2:-1:0:0", "f:10007:3:1:3:1", "d:1009:3:-1:1:0", "d:1031:2:1:2:1", "f:13:5:1:0:0",
+          "f:257:4:-1:3:0", "f:11x13:2:1:0:0", "d:17x2:3:-1:3:0"]
+
+
+def test_emulated_bluestein_two_workgroups():
+    """Bluestein over two workgroups per transform (the small-batch form), forced on for
+    every transform size, in both precisions and directions, every normalization,
+    in place and out of place, and through pointer arrays."""
+    _run(HALVES, {"SYCLFFT_HIP_BLUE_HALVES": "1"})
+    _run(["f:10007:3:-1:3:0", "d:1009:3:1:1:1"], {"SYCLFFT_HIP_BLUE_HALVES": "1", "EMU_MANY": "1"})
+    _run(["f:1009:5:-1:3:0"], {"SYCLFFT_HIP_BLUE_HALVES": "1", "EMU_MANY": "2"})

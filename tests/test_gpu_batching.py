@@ -43,13 +43,17 @@ CASES = [
     ([256, 13], 3, True, 1, "none", False, None),
     ([32, 32, 32], 3, False, 1, "none", True, None),
     ([64, 48, 10], 2, True, -1, "forward", False, None),
+    ([10007], 3, False, -1, "orthogonal", False, "halves"),
+    ([1009], 4, True, 1, "none", True, "halves"),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])) + f"_b{c[1]}" + ("_d" if c[2] else "_f") + (f"_{c[6]}" if c[6] else ""))
 def test_execute_many_matches_contiguous(case, monkeypatch):
     lengths, batch, double, direction, norm, in_place, tune = case
-    if tune:
+    if tune == "halves":  # Bluestein over two workgroups per transform
+        monkeypatch.setenv("SYCLFFT_HIP_BLUE_HALVES", "1")
+    elif tune:
         monkeypatch.setenv("SYCLFFT_HIP_TUNE", tune)
     n = int(np.prod(lengths))
     say(f"case lengths={lengths} batch={batch} double={double} dir={direction} norm={norm} "
