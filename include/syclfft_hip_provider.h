@@ -134,7 +134,10 @@ SYCLFFT_HIP_API uint64_t syclfft_hip_plan_scratch_bytes(const void *plan);
  * once the kernels are enqueued.  in/out are device pointers to batch*prod(lengths)
  * interleaved complex values; in == out is required for in-place plans and rejected for
  * out-of-place plans (src/sycl/sycl.cpp:619-635).  Replaces impl::execute
- * (src/sycl/sycl.cpp:615-776) and cufft_provider_v2::execute. */
+ * (src/sycl/sycl.cpp:615-776) and cufft_provider_v2::execute.
+ * May be captured into a HIP graph (kernel nodes only); under capture the ordering after
+ * the plan's previous execution is left to the captured stream order, and the plan must
+ * outlive the graph. */
 SYCLFFT_HIP_API int syclfft_hip_plan_execute(
     void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
     void *done, char *err, size_t cap);
@@ -146,7 +149,9 @@ SYCLFFT_HIP_API int syclfft_hip_plan_execute(
  * returns.  One plan then serves any number of independent buffers up to its batch count
  * in one set of launches (the serving loop batches requests this way).  No reference
  * counterpart: the reference executes one contiguous batch per call
- * (src/sycl/sycl.cpp:615-776). */
+ * (src/sycl/sycl.cpp:615-776).  Under HIP graph capture only contiguous pointers
+ * (ins[k] = ins[0] + k * prod(lengths) elements, likewise outs) are accepted; others
+ * need a table upload and return INVALID_STATE. */
 SYCLFFT_HIP_API int syclfft_hip_plan_execute_many(
     void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *const *deps,
     size_t ndeps, void *done, char *err, size_t cap);

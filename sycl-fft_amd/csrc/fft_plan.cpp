@@ -1794,6 +1794,19 @@ static plan *create_plan(const syclfft_hip_plan_config_v1 *cfg)
 // Device-pointer check: the whole [ptr, ptr+bytes) range must lie in one device
 // allocation visible to the plan's device (replaces the USM check of validate_pointer,
 // src/sycl/sycl.cpp:72-85).  A host pointer would fault the GPU, so it is rejected here.
+// Under stream capture (hipGraph) the plan's last-execution event is neither waited for
+// nor recorded: it would tie the graph to work outside it, and the captured stream order
+// already orders the plan's executions inside the graph.
+static bool stream_capturing(hipStream_t stream)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st == hipStreamCaptureStatusActive;
+}
+
 // Returns the allocation range [*lo, *hi) when the runtime reports it (0, 0 otherwise).
 static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, const char *name,
                              uintptr_t *lo = nullptr, uintptr_t *hi = nullptr)
@@ -1862,12 +1875,13 @@ static void execute_plan(
     }
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
+    const bool capturing = stream_capturing(stream);
     for (size_t i = 0; i < ndeps; ++i) {
         if (deps[i]) {
             hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
         }
     }
-    if (p.have_last) {
+    if (p.have_last && !capturing) {
         hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
     }
     if (p.copy_only) {
@@ -1885,8 +1899,10 @@ static void execute_plan(
             static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, l.grid, stream)),
             "tile kernel launch");
     }
-    hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
-    p.have_last = true;
+    if (!capturing) {
+        hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+        p.have_last = true;
+    }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
     }
@@ -1970,6 +1986,13 @@ static void execute_units(
     }
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
+    const bool capturing = stream_capturing(stream);
+    if (capturing && !affine) {
+        // a captured copy would read the host staging slot when the graph replays, after
+        // later executions may have refilled it
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE,
+                        "pointer arrays that need unit tables cannot be captured into a graph");
+    }
     if (affine) {
         cap_entries = 0;
     }
@@ -2026,7 +2049,7 @@ static void execute_units(
             hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
         }
     }
-    const bool serialize = ordered || p.scratch != nullptr;
+    const bool serialize = (ordered || p.scratch != nullptr) && !capturing;
     if (serialize && p.have_last) {
         hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
     }

@@ -245,6 +245,11 @@ hipError_t hipStreamCreate(hipStream_t *s)
     *s = nullptr;
     return hipSuccess;
 }
+hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus *st)
+{
+    *st = hipStreamCaptureStatusNone; // the emulation has no graphs
+    return hipSuccess;
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
 {
     *s = nullptr;

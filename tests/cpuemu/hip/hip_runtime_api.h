@@ -99,6 +99,12 @@ hipError_t hipDeviceGetAttribute(int *, hipDeviceAttribute_t, int);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *, const void *, int, size_t);
 hipError_t hipEventQuery(hipEvent_t);
 hipError_t hipStreamCreate(hipStream_t *);
+typedef enum hipStreamCaptureStatus
+{
+    hipStreamCaptureStatusNone = 0,
+    hipStreamCaptureStatusActive = 1
+} hipStreamCaptureStatus;
+hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus *);
 #define hipStreamNonBlocking 1u
 hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
