@@ -19,8 +19,12 @@ the reference-algorithm CPU baseline (the oracle's restatement of the portable p
 OpenMP on the host cores) on a bounded sample, and the serving loop (batched capacity,
 isolated latency, p50/p99 latency at 80% of capacity) on a seeded mixed request stream.
 
+Also: the precision configuration (fp32 N=16384 batch 4096 and N=2^20 batch 16: plain FP32
+VALU kernel vs radix-16 passes on MFMA with split BF16 / scaled split FP16 operands, time and
+rel-L2 against the FP64 oracle per variant).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep] [--no-overlap]
-                       [--no-serve] [--serve-requests R]
+                       [--no-serve] [--serve-requests R] [--no-precision]
 """
 import argparse
 import ctypes
@@ -246,6 +250,57 @@ def multid_bench(sh, torch, dev, reps=10):
     return out
 
 
+def precision_bench(sh, torch, dev, lib, reps=20):
+    """The precision configuration: batched fp32 C2C N=16384 batch 4096 and N=2^20 batch 16,
+    plain FP32 VALU kernel vs radix-16 passes on MFMA with split BF16 (hi/lo, three products)
+    and scaled split FP16 operands, FP32 accumulation.  Per variant: time per execute (HIP
+    events over back-to-back executions), GFLOP/s, algorithmic GB/s, speed-up over the VALU
+    kernel, and rel-L2 against the FP64 oracle on sampled lines for uniform[-1,1) inputs and
+    for log-uniform magnitudes 1e-4..1e4 with uniform phase."""
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(20260003)
+    for name, n, b in (("c4_f32_16384x4096", 16384, 4096), ("c4_f32_1048576x16", 1 << 20, 16)):
+        x = torch.view_as_complex((torch.rand((n * b, 2), generator=g, device=dev) * 2 - 1).contiguous())
+        mag = 10.0 ** (torch.rand(n * b, generator=g, device=dev) * 8 - 4)
+        xl = torch.polar(mag, torch.rand(n * b, generator=g, device=dev) * (2 * np.pi))
+        y = torch.empty_like(x)
+        lines = 2 if n <= 16384 else 1
+        refs = {}
+        for label, inp in (("uniform", x), ("log_uniform", xl)):
+            xs = inp[: n * lines].cpu().numpy()
+            refs[label] = (xs, oracle_fft(lib, xs, n, lines, -1, 0))
+        res = {}
+        for prec in ("valu", "split_bf16", "split_fp16"):
+            p = sh.Plan([n], b, sh.FORWARD, precision=prec, measure=True)
+            for _ in range(2):
+                p.execute(x, y)
+            a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                p.execute(x, y)
+            b_.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b_) / reps
+            r = {"ms": round(ms, 4), "gflops": round(flops(n, b) / (ms * 1e-3) / 1e9, 1),
+                 "hbm_gbs": round(2 * n * b * 8 / (ms * 1e-3) / 1e9, 1), "launches": p.launch_count(),
+                 "mfma_launches": p.describe().count("split=")}
+            for label, inp in (("uniform", x), ("log_uniform", xl)):
+                p.execute(inp, y)
+                torch.cuda.synchronize()
+                ys = y[: n * lines].cpu().numpy().astype(np.complex128)
+                ref = refs[label][1]
+                r["rel_l2_" + label] = float(np.linalg.norm(ys - ref) / np.linalg.norm(ref))
+            res[prec] = r
+            p.close()
+        for prec in ("split_bf16", "split_fp16"):
+            res[prec]["speedup_vs_valu"] = round(res["valu"]["ms"] / res[prec]["ms"], 3)
+        out[name] = res
+        del x, xl, y, mag
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +311,7 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="skip the three-stream report")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--no-precision", action="store_true", help="skip the split-precision MFMA comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every transform of this workload fits one MI355X and "
@@ -440,6 +496,11 @@ def main():
     # ---------------- multi-D configuration
     multid = None if args.no_sweep else multid_bench(sh, torch, dev)
 
+    # ---------------- precision configuration: plain FP32 VALU vs split-precision MFMA
+    precision = None if args.no_precision else precision_bench(sh, torch, dev, lib)
+    if precision:
+        log(f"precision: {precision}")
+
     # ---------------- serving loop (continuous batching over resident plans)
     serving = None
     if not args.no_serve:
@@ -487,6 +548,7 @@ def main():
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
         "multid": multid,
+        "precision": precision,
         "serving": serving,
     }
     print(json.dumps(line))

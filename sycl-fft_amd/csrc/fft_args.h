@@ -42,7 +42,11 @@ enum sfh_mode : int
     SFH_M_PRE = 1,  // multiply loaded element j by pre[p], zero-fill p >= nvalid_in
     SFH_M_MID = 2,  // after FFT #1: multiply by mid[s], conjugate, run FFT #2
     SFH_M_T4 = 4,   // after the last FFT: multiply output k by W_N^(col * k)
-    SFH_M_POST = 8  // multiply stored element by post[q], skip q >= nvalid_out
+    SFH_M_POST = 8, // multiply stored element by post[q], skip q >= nvalid_out
+    // fp32 plans: radix-16 passes as 16-point DFT matrix products on MFMA with split
+    // operands (hi + lo, three products, FP32 accumulation) instead of VALU butterflies
+    SFH_M_SPLIT_BF16 = 16, // BF16 hi/lo
+    SFH_M_SPLIT_F16 = 32   // FP16 hi/lo, each butterfly scaled by a power of two
 };
 
 // Run-time flags.

@@ -49,7 +49,9 @@ std::once_flag g_once;
 void allow_lds(sfh_kernel_variant &v)
 {
     for (const void *fn : v.fn) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        if (fn) {
+            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        }
     }
     (void)hipGetLastError();
 }
@@ -98,6 +100,9 @@ extern "C" int sfh_device_cus()
 extern "C" int sfh_resident_groups(const sfh_kernel_variant *variant, int mode_index, uint32_t lds_bytes)
 {
     int n = 0;
+    if (!variant->fn[mode_index]) {
+        return 0;
+    }
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, variant->fn[mode_index], variant->nt, lds_bytes)
         != hipSuccess) {
         (void)hipGetLastError();
@@ -115,7 +120,7 @@ extern "C" int sfh_launch_tile(
     }
     // AQL dispatches carry the grid size in work-items as a 32-bit value
     if (grid == 0 || grid > args->ntiles || static_cast<uint64_t>(grid) * variant->nt > 0xffffffffull
-        || mode_index < 0 || mode_index >= SFH_NUM_MODES) {
+        || mode_index < 0 || mode_index >= SFH_NUM_MODES || !variant->fn[mode_index]) {
         return static_cast<int>(hipErrorInvalidConfiguration);
     }
     void *kernel_args[] = {const_cast<sfh_tile_args *>(args)};

@@ -16,12 +16,16 @@ enum sfh_mode_index
     SFH_MI_BLUE_C = 5,    // column FFT, chirp, truncate
     SFH_MI_PLAIN_PF = 6,  // persistent, prefetching form of SFH_MI_PLAIN
     SFH_MI_T4_PF = 7,     // persistent, prefetching form of SFH_MI_T4
-    SFH_NUM_MODES = 8
+    SFH_MI_PLAIN_BF16 = 8,// SFH_MI_PLAIN with radix-16 passes on MFMA, split BF16 operands
+    SFH_MI_T4_BF16 = 9,   // SFH_MI_T4, same
+    SFH_MI_PLAIN_F16 = 10,// SFH_MI_PLAIN with radix-16 passes on MFMA, scaled split FP16
+    SFH_MI_T4_F16 = 11,   // SFH_MI_T4, same
+    SFH_NUM_MODES = 12
 };
 
 struct sfh_kernel_variant
 {
-    const void *fn[SFH_NUM_MODES]; // __global__ entries per fusion mode
+    const void *fn[SFH_NUM_MODES]; // __global__ entries per fusion mode (null: not compiled)
     int nt;                        // threads per workgroup
     int e;                         // elements per thread; LDS capacity = nt * e complex values
     int maxr;                      // largest radix compiled into the variant (16, or 9 when e > 16)

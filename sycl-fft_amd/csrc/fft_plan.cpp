@@ -269,6 +269,7 @@ struct plan
     int nvariants{};
     int copy_only{}; // all lengths 1: a scaled copy
     int cap_limit{};  // largest tile (complex elements) the planner may use; 0 = no limit
+    int split_launches{}; // launches whose radix-16 passes run on MFMA (split precision)
     // unit-table ring for executions over pointer arrays (allocated on first use): per slot
     // a device table block, its pinned host staging copy, and the event after which the
     // slot's last execution no longer reads it
@@ -1490,6 +1491,32 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
             const double sc = final_write ? p->scale : 1.0;
             l.args.out_sx = sc;
             l.args.out_sy = (conj_out != l.conj_post) ? -sc : sc;
+        }
+    }
+
+    // Split-precision MFMA forms (fp32 plans, precision != VALU): every plain or four-step
+    // launch with radix-16 passes on a variant that compiles them (fp32, 16 elements per
+    // lane, whole waves) runs its radix-16 passes on MFMA.  Other passes (radix 2/3/4/5/7/8/9)
+    // and the Bluestein launches stay on the VALU.
+    if (cfg->precision != SYCLFFT_HIP_PRECISION_VALU) {
+        const bool bf = cfg->precision == SYCLFFT_HIP_PRECISION_SPLIT_BF16;
+        for (auto &l : p->launches) {
+            if (l.mode != SFH_MI_PLAIN && l.mode != SFH_MI_T4) {
+                continue;
+            }
+            const sfh_kernel_variant &v = p->variants[l.variant];
+            const int sm = (l.mode == SFH_MI_PLAIN ? SFH_MI_PLAIN_BF16 : SFH_MI_T4_BF16) + (bf ? 0 : 2);
+            const sfh_tile_args &a = l.args;
+            bool radix16 = false;
+            for (uint32_t k = 0; k < a.npass; ++k) {
+                radix16 = radix16 || a.pass[k].radix == 16;
+            }
+            if (!v.fn[sm] || v.e != 16 || !radix16 || v.nt % 64 != 0) {
+                continue;
+            }
+            l.mode = sm;
+            l.desc += bf ? " split=bf16x3" : " split=fp16x3";
+            ++p->split_launches;
         }
     }
 

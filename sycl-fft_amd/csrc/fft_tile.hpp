@@ -344,6 +344,9 @@ struct tile_ctx
     const cx<T> *tw;   // pass-twiddle tables in LDS
     const cx<T> *t4;   // four-step tables in LDS
     bool wvalid;       // this lane's column exists in the tile
+    // split-precision modes: this lane's A operands of the 16-point DFT matrix (packed
+    // pairs of BF16/FP16 values; words 8h..8h+3 = hi, 8h+4..8h+7 = lo of output half h)
+    uint32_t amat[16];
 };
 
 __device__ __forceinline__ uint32_t col_of(const sfh_tile_args &a, uint32_t w0, uint32_t blo, uint32_t w)
@@ -519,6 +522,399 @@ struct no_hook
     __device__ void operator()() const {}
 };
 
+// ---------------------------------------------------------------------------------
+// Split-precision radix-16 passes on MFMA (fp32 plans, modes SFH_M_SPLIT_BF16/_F16).
+//
+// A radix-16 butterfly is the real 32x32 product [Re y; Im y] = F [Re x; Im x] with the
+// 16-point DFT matrix F, re/im interleaved along both dimensions.  A wave computes 16
+// butterflies at once with two v_mfma_f32_16x16x32 products (output rows 0..15 = outputs
+// 0..7, rows 16..31 = outputs 8..15):
+//   A (16 x 32): lane l holds row l%16, columns 8(l/16) .. 8(l/16)+7
+//   B (32 x 16): lane l holds column l%16 (its butterfly), rows 8(l/16) .. 8(l/16)+7,
+//                i.e. inputs t = 4(l/16) .. 4(l/16)+3 of that butterfly, re/im pairs
+//   D (16 x 16): lane l holds column l%16, rows 4(l/16) .. 4(l/16)+3, i.e. outputs
+//                2(l/16) and 2(l/16)+1 (and 8 more in the second product)
+// so each lane loads, twiddles and stores four of its butterfly's sixteen points, exactly
+// the LDS/HBM slots the VALU pass uses for them.  Operands are split x = hi + lo in BF16
+// or FP16 and the product is A_lo B_hi + A_hi B_lo + A_hi B_hi (the lo*lo term is dropped),
+// accumulated in FP32.  FP16 operands are scaled per butterfly by 2^(15 - e), e the
+// binary exponent of the butterfly's largest component, so hi never overflows and lo keeps
+// 11 more bits; the outputs are scaled back exactly.
+// ---------------------------------------------------------------------------------
+
+constexpr int split_prec(int mode)
+{
+    return (mode & SFH_M_SPLIT_BF16) ? 1 : ((mode & SFH_M_SPLIT_F16) ? 2 : 0);
+}
+
+#if defined(__HIP__)
+typedef float sfh_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t sfh_u4 __attribute__((ext_vector_type(4)));
+typedef __bf16 sfh_bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 sfh_h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 sfh_bf2 __attribute__((ext_vector_type(2)));
+typedef _Float16 sfh_h2 __attribute__((ext_vector_type(2)));
+
+// two fp32 values rounded to the operand type (nearest even), element 0 in the low half
+template <int P>
+__device__ __forceinline__ uint32_t split_pack(float a, float b)
+{
+    if constexpr (P == 1) {
+        const sfh_bf2 v = {static_cast<__bf16>(a), static_cast<__bf16>(b)};
+        return __builtin_bit_cast(uint32_t, v);
+    } else {
+        const sfh_h2 v = {static_cast<_Float16>(a), static_cast<_Float16>(b)};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+}
+template <int P>
+__device__ __forceinline__ float split_elem0(uint32_t w)
+{
+    if constexpr (P == 1) {
+        return __uint_as_float(w << 16);
+    } else {
+        return static_cast<float>(__builtin_bit_cast(sfh_h2, w).x);
+    }
+}
+template <int P>
+__device__ __forceinline__ float split_elem1(uint32_t w)
+{
+    if constexpr (P == 1) {
+        return __uint_as_float(w & 0xffff0000u);
+    } else {
+        return static_cast<float>(__builtin_bit_cast(sfh_h2, w).y);
+    }
+}
+// acc += A B for one 16x16x32 block (a, b: this lane's four packed operand words)
+template <int P>
+__device__ __forceinline__ void mfma16(sfh_f4 &acc, const uint32_t *a, const uint32_t *b)
+{
+    const sfh_u4 av = {a[0], a[1], a[2], a[3]};
+    const sfh_u4 bv = {b[0], b[1], b[2], b[3]};
+    if constexpr (P == 1) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(sfh_bf8, av), __builtin_bit_cast(sfh_bf8, bv), acc, 0, 0, 0);
+    } else {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            __builtin_bit_cast(sfh_h8, av), __builtin_bit_cast(sfh_h8, bv), acc, 0, 0, 0);
+    }
+}
+__device__ __forceinline__ float wave_xor_f(float v, int mask)
+{
+    return __shfl_xor(v, mask);
+}
+__device__ __forceinline__ int exponent_of(float v)
+{
+    return __builtin_amdgcn_frexp_expf(v);
+}
+__device__ __forceinline__ float scale2(float v, int e)
+{
+    return ldexpf(v, e);
+}
+__device__ __forceinline__ uint32_t wave_lane()
+{
+    return __lane_id();
+}
+#else
+// host build (CPU emulation): operands as bit patterns, MFMA and lane exchange emulated
+struct sfh_f4
+{
+    float v[4];
+    float &operator[](int i)
+    {
+        return v[i];
+    }
+    float operator[](int i) const
+    {
+        return v[i];
+    }
+};
+template <int P>
+inline uint32_t split_pack(float a, float b)
+{
+    return P == 1 ? (emu_to_bf16(a) | (emu_to_bf16(b) << 16)) : (emu_to_f16(a) | (emu_to_f16(b) << 16));
+}
+template <int P>
+inline float split_elem0(uint32_t w)
+{
+    return P == 1 ? emu_from_bf16(w & 0xffffu) : emu_from_f16(w & 0xffffu);
+}
+template <int P>
+inline float split_elem1(uint32_t w)
+{
+    return P == 1 ? emu_from_bf16(w >> 16) : emu_from_f16(w >> 16);
+}
+template <int P>
+inline void mfma16(sfh_f4 &acc, const uint32_t *a, const uint32_t *b)
+{
+    emu_mfma_16x16x32(P, acc.v, a, b);
+}
+inline float wave_xor_f(float v, int mask)
+{
+    return emu_wave_xor(v, mask);
+}
+inline int exponent_of(float v)
+{
+    int e = 0;
+    (void)std::frexp(v, &e);
+    return e;
+}
+inline float scale2(float v, int e)
+{
+    return std::ldexp(v, e);
+}
+inline uint32_t wave_lane()
+{
+    return threadIdx.x & 63u;
+}
+#endif
+
+// cos(2 pi p / 16) for any p (bits 0..3 used)
+__device__ __forceinline__ float cos16(uint32_t p)
+{
+    const float k1 = 0.92387953251128675613f; // cos(pi/8)
+    const float k2 = 0.70710678118654752440f; // cos(pi/4)
+    const float k3 = 0.38268343236508977173f; // cos(3pi/8)
+    const uint32_t q = p & 7u;
+    const float v = q == 0u ? 1.0f
+                  : q == 1u ? k1
+                  : q == 2u ? k2
+                  : q == 3u ? k3
+                  : q == 4u ? 0.0f
+                  : q == 5u ? -k3
+                  : q == 6u ? -k2
+                            : -k1;
+    return (p & 8u) ? -v : v;
+}
+
+// This lane's A operands of F (forward, y_k = sum_t x_t W16^(k t)): row r = lane%16 of
+// output half h is output k = 8h + r/2, real part for even r, imaginary for odd r;
+// columns 8g + jj are input t = 4g + jj/2, real part for even jj, imaginary for odd jj.
+//   Re y += C Re x + S Im x,   Im y += -S Re x + C Im x,   (C, S) = (cos, sin)(2 pi k t / 16)
+template <int P>
+__device__ __forceinline__ void split_dft16_matrix(uint32_t *amat, uint32_t lane)
+{
+    const uint32_t r = lane & 15u, g = lane >> 4;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t k = 8u * h + (r >> 1);
+        float vals[8];
+#pragma unroll
+        for (uint32_t jj = 0; jj < 8; ++jj) {
+            const uint32_t t = 4u * g + (jj >> 1);
+            const uint32_t e = (k * t) & 15u;
+            const float C = cos16(e), S = cos16(e + 12u); // sin x = cos(x - pi/2)
+            vals[jj] = (r & 1u) ? ((jj & 1u) ? C : -S) : ((jj & 1u) ? S : C);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t hi = split_pack<P>(vals[2 * j], vals[2 * j + 1]);
+            amat[8 * h + j] = hi;
+            amat[8 * h + 4 + j] =
+                split_pack<P>(vals[2 * j] - split_elem0<P>(hi), vals[2 * j + 1] - split_elem1<P>(hi));
+        }
+    }
+}
+
+// Butterfly of a split-precision group: the VALU lane mapping applied to the virtual lane
+// vt, so group q of a wave takes the butterflies VALU lanes 16q .. 16q+15 would own.
+struct split_bfly
+{
+    uint32_t i; // butterfly index in its column
+    uint32_t w; // column in the tile
+    bool ok;    // column exists and i < m
+};
+__device__ __forceinline__ split_bfly split_butterfly(const sfh_tile_args &a, uint32_t wv, uint32_t vt, uint32_t m)
+{
+    split_bfly b;
+    if (a.W == 1) {
+        b.w = 0;
+        b.i = vt;
+    } else if (a.flags & SFH_WFAST) {
+        b.w = vt & (a.W - 1);
+        b.i = vt >> a.logW;
+    } else {
+        b.w = vt >> a.logtpc;
+        b.i = vt & (a.tpc - 1);
+    }
+    b.ok = b.w < wv && b.i < m;
+    return b;
+}
+
+// One radix-16 Stockham pass of a split-precision launch.  The wave's 64 VALU-mode
+// butterflies (one per lane: E = 16 = R) run as four groups of 16: group q takes those of
+// VALU lanes 16q .. 16q+15, and lane l works on four of the sixteen points of butterfly l % 16.
+template <typename T, int MODE, class Hook>
+__device__ __forceinline__ void run_pass16_split(
+    const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
+    int out_mode, Hook after_loads)
+{
+    static_assert(sizeof(T) == 4, "split-precision passes are for fp32 plans");
+    constexpr int P = split_prec(MODE);
+    const uint32_t m = a.L >> 4; // butterflies per column
+    const uint32_t lane = wave_lane();
+    const uint32_t g = lane >> 4;
+    const uint32_t vt0 = threadIdx.x - lane + (lane & 15u);
+    const bool col_units = a.unit_dim == SFH_UNIT_COL;
+    cx<T> v[4][4];
+
+    if (in_mode == IN_GLOBAL) {
+        const int64_t tstep = static_cast<int64_t>(m) * a.in_j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const split_bfly b = split_butterfly(a, c.wv, vt0 + 16u * q, m);
+            if (b.ok) {
+                const cx<T> *src = (col_units && a.in_units)
+                                       ? static_cast<const cx<T> *>(a.in_units[c.w0 + b.w])
+                                       : c.tsrc + static_cast<int32_t>(b.w) * a.in_w;
+                const cx<T> *bq = src + static_cast<int64_t>(b.i) * a.in_j + static_cast<int64_t>(4u * g) * tstep;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[q][j] = bq[j * tstep];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[q][j] = {T(0), T(0)};
+                }
+            }
+        }
+        after_loads();
+        if (a.flags & SFH_CONJ_IN) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[q][j].y = -v[q][j].y;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const split_bfly b = split_butterfly(a, c.wv, vt0 + 16u * q, m);
+            if (b.ok) {
+                const cx<T> *ldsc = lds + b.w * a.LP;
+                const uint32_t i = b.i;
+                const uint32_t gi = lds_slot(i, p.pad_in);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t tm = (4u * g + static_cast<uint32_t>(j)) * m;
+                    v[q][j] = ldsc[gi + tm + (p.pad_in ? (tm >> 5) : 0u)];
+                }
+                if (p.ns > 1) {
+                    // W^(step t) for t = 4g + j: W^(4g step) W^(j step)
+                    const uint32_t kq = fdiv(i, p.ns_div);
+                    const uint32_t step = (i - kq * p.ns) * p.twstride;
+                    const cx<T> w1 = pass_twiddle(c.tw, a.tw_bits, step);
+                    const cx<T> w2 = pass_twiddle(c.tw, a.tw_bits, 2u * step);
+                    const cx<T> wb = pass_twiddle(c.tw, a.tw_bits, 4u * g * step);
+                    const cx<T> wb2 = cmul(wb, w2);
+                    v[q][0] = cmul(v[q][0], wb);
+                    v[q][1] = cmul(v[q][1], cmul(wb, w1));
+                    v[q][2] = cmul(v[q][2], wb2);
+                    v[q][3] = cmul(v[q][3], cmul(wb2, w1));
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[q][j] = {T(0), T(0)};
+                }
+            }
+        }
+    }
+
+    // 16-point DFTs: two output halves x three split products per group
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x[2 * j] = v[q][j].x;
+            x[2 * j + 1] = v[q][j].y;
+        }
+        int ex = 0;
+        if constexpr (P == 2) {
+            float mx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                mx = fmaxf(mx, fabsf(x[k]));
+            }
+            mx = fmaxf(mx, wave_xor_f(mx, 16));
+            mx = fmaxf(mx, wave_xor_f(mx, 32));
+            ex = exponent_of(mx); // mx < 2^ex
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                x[k] = scale2(x[k], 15 - ex);
+            }
+        }
+        uint32_t bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bh[j] = split_pack<P>(x[2 * j], x[2 * j + 1]);
+            bl[j] = split_pack<P>(x[2 * j] - split_elem0<P>(bh[j]), x[2 * j + 1] - split_elem1<P>(bh[j]));
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            sfh_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            mfma16<P>(acc, c.amat + 8 * h + 4, bh); // A_lo B_hi
+            mfma16<P>(acc, c.amat + 8 * h, bl);     // A_hi B_lo
+            mfma16<P>(acc, c.amat + 8 * h, bh);     // A_hi B_hi
+            if constexpr (P == 2) {
+                v[q][2 * h] = {scale2(acc[0], ex - 15), scale2(acc[1], ex - 15)};
+                v[q][2 * h + 1] = {scale2(acc[2], ex - 15), scale2(acc[3], ex - 15)};
+            } else {
+                v[q][2 * h] = {acc[0], acc[1]};
+                v[q][2 * h + 1] = {acc[2], acc[3]};
+            }
+        }
+    }
+    // v[q][j] now holds output 2g + (j & 1) + 8 (j >> 1) of group q's butterfly
+
+    if (out_mode == OUT_GLOBAL) {
+        const epilogue<T> ep(a);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const split_bfly b = split_butterfly(a, c.wv, vt0 + 16u * q, m);
+            if (b.ok) {
+                cx<T> *dst = (col_units && a.out_units) ? static_cast<cx<T> *>(a.out_units[c.w0 + b.w])
+                                                        : c.tdst + static_cast<int32_t>(b.w) * a.out_w;
+                uint32_t col = 0;
+                if constexpr (MODE & SFH_M_T4) {
+                    col = col_of(a, c.w0, c.blo, b.w);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t k = b.i + (2u * g + (j & 1) + 8u * (j >> 1)) * m;
+                    cx<T> y = v[q][j];
+                    if constexpr (MODE & SFH_M_T4) {
+                        y = cmul(y, t4_twiddle(a, c.t4, col * k));
+                    }
+                    dst[static_cast<int64_t>(k) * a.out_k] = ep(y);
+                }
+            }
+        }
+        return;
+    }
+
+    __syncthreads(); // every lane has read this pass's inputs
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const split_bfly b = split_butterfly(a, c.wv, vt0 + 16u * q, m);
+        if (b.ok) {
+            cx<T> *ldsc = lds + b.w * a.LP;
+            const uint32_t kq = fdiv(b.i, p.ns_div);
+            const uint32_t gb = lds_slot(kq * p.ns * 15u + b.i, p.pad_out);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t ts = (2u * g + (j & 1) + 8u * (j >> 1)) * p.ns;
+                ldsc[gb + ts + (p.pad_out ? (ts >> 5) : 0u)] = v[q][j];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // after_loads() runs once every lane has issued its pass-0 HBM loads (IN_GLOBAL), before
 // they are consumed: the twiddle tables are staged there so both round trips overlap.
 template <typename T, int NT, int E, int R, int MODE, class Hook = no_hook>
@@ -526,6 +922,10 @@ __device__ __forceinline__ void run_pass(
     const sfh_tile_args &a, const tile_ctx<T> &c, cx<T> *lds, const sfh_pass &p, int in_mode,
     int out_mode, bool wvalid, const cx<T> *pf, Hook after_loads = Hook())
 {
+    if constexpr (split_prec(MODE) != 0 && R == 16) {
+        run_pass16_split<T, MODE>(a, c, lds, p, in_mode, out_mode, after_loads);
+        return;
+    }
     constexpr int NB = (E + R - 1) / R;
     const uint32_t m = a.L / R; // butterflies per column
     cx<T> *ldsc = lds + c.w * a.LP;
@@ -988,7 +1388,7 @@ __device__ __forceinline__ void set_tile(const sfh_tile_args &a, tile_ctx<T> &c,
             c.dst = static_cast<cx<T> *>(a.out_units[col]);
         }
     }
-    c.col = (MODE != 0) ? col_of(a, c.w0, c.blo, c.w) : 0u;
+    c.col = (MODE & (SFH_M_PRE | SFH_M_MID | SFH_M_T4 | SFH_M_POST)) ? col_of(a, c.w0, c.blo, c.w) : 0u;
 }
 
 // Waves per SIMD that the LDS footprint allows; used as the register-budget hint.
@@ -1039,6 +1439,9 @@ void tile_fft(const sfh_tile_args a_byval)
         __syncthreads();
     };
 
+    if constexpr (split_prec(MODE) != 0) {
+        split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
+    }
     if constexpr (!PREF) {
         // one tile per workgroup
         set_tile<T, MODE>(a, c, blockIdx.x, tid);

@@ -1,6 +1,7 @@
 // Runs provider plans on the CPU emulation and checks them against the FP64 oracle
 // (TEST INFRASTRUCTURE ONLY).  Usage: emu_driver <case>...  with a case written as
-//   f|d:n0[xn1[xn2]]:batch:dir:norm:inplace      e.g.  f:4096:2:-1:0:0
+//   f|d|b|h:n0[xn1[xn2]]:batch:dir:norm:inplace      e.g.  f:4096:2:-1:0:0
+// (b, h: float plans with split-precision BF16 / FP16 MFMA radix-16 passes)
 // Prints one line per case: "ok <case> relL2=<e>" or "FAIL <case> ..."; exit status 1 on
 // any failure.  EMU_MANY=1 executes through syclfft_hip_plan_execute_many with every
 // transform in its own allocation (allocated in reverse order); EMU_MANY=2 does the same for
@@ -36,7 +37,12 @@ static bool run_case(const std::string &spec)
     for (char *tok = std::strtok(dims, "x"); tok; tok = std::strtok(nullptr, "x")) {
         lens.push_back(std::strtoull(tok, nullptr, 10));
     }
+    // 'f' float, 'd' double, 'b' / 'h': float with split-precision MFMA radix-16 passes
+    // (BF16 / scaled FP16 hi+lo operands)
     const bool dbl = prec == 'd';
+    const uint32_t split = prec == 'b' ? SYCLFFT_HIP_PRECISION_SPLIT_BF16
+                         : prec == 'h' ? SYCLFFT_HIP_PRECISION_SPLIT_FP16
+                                       : SYCLFFT_HIP_PRECISION_VALU;
     syclfft_hip_plan_config_v1 cfg{};
     cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
     cfg.scalar = dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
@@ -50,6 +56,7 @@ static bool run_case(const std::string &spec)
     cfg.normalization = static_cast<uint32_t>(norm);
     cfg.device = -1;
     cfg.measure = std::getenv("EMU_MEASURE") ? 1u : 0u;
+    cfg.precision = split;
     char err[1024];
     void *plan = nullptr;
     const int st = syclfft_hip_plan_create_ex(&cfg, &plan, err, sizeof(err));
@@ -159,7 +166,12 @@ static bool run_case(const std::string &spec)
     for (auto n : lens) {
         tsize *= n;
     }
-    const double u_ = dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    // unit roundoff of the arithmetic: split BF16 operands carry 16 bits (hi + lo, lo*lo
+    // dropped), scaled split FP16 operands 22 bits
+    const double u_ = dbl ? std::ldexp(1.0, -53)
+                    : split == SYCLFFT_HIP_PRECISION_SPLIT_BF16 ? std::ldexp(1.0, -16)
+                    : split == SYCLFFT_HIP_PRECISION_SPLIT_FP16 ? std::ldexp(1.0, -22)
+                                                                 : std::ldexp(1.0, -24);
     const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(tsize))) * u_;
     char desc[4096];
     syclfft_hip_plan_describe(plan, desc, sizeof(desc));

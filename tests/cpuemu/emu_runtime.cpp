@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <barrier>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -39,6 +40,104 @@ unsigned char *emu_dynamic_lds()
 void emu_syncthreads()
 {
     g_bar->arrive_and_wait();
+}
+
+uint32_t emu_to_bf16(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    if (std::isnan(v)) {
+        return 0x7fc0u;
+    }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+float emu_from_bf16(uint32_t h)
+{
+    const uint32_t u = h << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+uint32_t emu_to_f16(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const float a = std::fabs(v);
+    if (std::isnan(v)) {
+        return sign | 0x7e00u;
+    }
+    if (a >= 65520.0f) {
+        return sign | 0x7c00u;
+    }
+    if (a < 6.103515625e-05f) { // subnormal: a multiple of 2^-24
+        return sign | static_cast<uint32_t>(std::nearbyint(a * 16777216.0f));
+    }
+    uint32_t au;
+    std::memcpy(&au, &a, 4);
+    au += 0xfffu + ((au >> 13) & 1u);
+    const uint32_t e = (au >> 23) - 127u + 15u;
+    return sign | (e << 10) | ((au >> 13) & 0x3ffu);
+}
+
+float emu_from_f16(uint32_t h)
+{
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0) {
+        v = std::ldexp(static_cast<float>(m), -24);
+    } else if (e == 31) {
+        v = m ? NAN : INFINITY;
+    } else {
+        v = std::ldexp(static_cast<float>(m | 0x400u), static_cast<int>(e) - 25);
+    }
+    return (h & 0x8000u) ? -v : v;
+}
+
+namespace
+{
+float g_xbuf[1024];
+uint32_t g_abuf[1024][4];
+uint32_t g_bbuf[1024][4];
+} // namespace
+
+float emu_wave_xor(float v, int mask)
+{
+    const unsigned t = threadIdx.x;
+    g_xbuf[t] = v;
+    emu_syncthreads();
+    const unsigned lane = t & 63u;
+    const float r = g_xbuf[(t - lane) + (lane ^ static_cast<unsigned>(mask))];
+    emu_syncthreads();
+    return r;
+}
+
+void emu_mfma_16x16x32(int prec, float *acc, const uint32_t *a, const uint32_t *b)
+{
+    const unsigned t = threadIdx.x;
+    std::memcpy(g_abuf[t], a, 16);
+    std::memcpy(g_bbuf[t], b, 16);
+    emu_syncthreads();
+    const unsigned lane = t & 63u, base = t - lane, n = lane & 15u, g = lane >> 4;
+    auto dec = [&](uint32_t w, unsigned half) {
+        const uint32_t bits = half ? (w >> 16) : (w & 0xffffu);
+        return prec == 1 ? emu_from_bf16(bits) : emu_from_f16(bits);
+    };
+    for (unsigned j = 0; j < 4; ++j) {
+        const unsigned row = 4 * g + j;
+        double s = acc[j];
+        for (unsigned k = 0; k < 32; ++k) {
+            const unsigned kk = k % 8;
+            const double av = dec(g_abuf[base + row + 16 * (k / 8)][kk / 2], kk % 2);
+            const double bv = dec(g_bbuf[base + n + 16 * (k / 8)][kk / 2], kk % 2);
+            s += av * bv;
+        }
+        acc[j] = static_cast<float>(s);
+    }
+    emu_syncthreads();
 }
 
 void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body)

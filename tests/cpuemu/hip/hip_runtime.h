@@ -4,6 +4,8 @@
 
 #include "hip_runtime_api.h"
 
+#include <cmath>
+#include <cstdint>
 #include <functional>
 
 #define __global__
@@ -22,6 +24,17 @@ void emu_syncthreads();
 // dynamic LDS of the running launch (sized by the launch's sharedMem argument)
 unsigned char *emu_dynamic_lds();
 #define __syncthreads() emu_syncthreads()
+
+// Wave-level operations of the split-precision MFMA passes.  Every work-item of the
+// workgroup calls them at the same program point (they synchronize the workgroup).
+// Operand words hold two BF16 (prec 1) or FP16 (prec 2) values, element 0 in the low half.
+uint32_t emu_to_bf16(float v);  // round to nearest even, bit pattern
+float emu_from_bf16(uint32_t h);
+uint32_t emu_to_f16(float v);   // round to nearest even (subnormals, overflow to inf)
+float emu_from_f16(uint32_t h);
+float emu_wave_xor(float v, int mask); // value of lane (lane ^ mask) of this wave of 64
+// acc[j] += sum_k A[4(l/16)+j][k] B[k][l%16], the v_mfma_f32_16x16x32 lane layout
+void emu_mfma_16x16x32(int prec, float *acc, const uint32_t *a, const uint32_t *b);
 
 inline unsigned __umulhi(unsigned a, unsigned b)
 {

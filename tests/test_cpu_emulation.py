@@ -47,6 +47,19 @@ def test_emulated_launch_kinds():
     _run(CASES)
 
 
+# split-precision MFMA plans (b: BF16 hi/lo, h: scaled FP16 hi/lo): the MFMA products and
+# the lane exchange are emulated with the v_mfma_f32_16x16x32 lane layout; shapes cover
+# W = 1, W > 1 with row-major and column-fast lanes, four-step A/B, multi-D
+SPLIT = ["b:16:3:-1:0:0", "h:16:3:1:3:0", "h:256:5:1:3:1", "b:4096:1:1:1:0", "h:4096:1:-1:0:0",
+         "h:64x1x8:3:1:0:0", "b:80x48:2:1:0:0", "h:256x16:3:1:1:0"]
+
+
+def test_emulated_split_precision():
+    _run(SPLIT)
+    _run(["h:4096:2:-1:0:0", "b:2048x96:1:-1:1:0"], {"SYCLFFT_HIP_MAX_TILE": "1024"})
+    _run(["h:4096:3:-1:0:0", "b:256x16:3:1:1:0"], {"EMU_MANY": "1"})
+
+
 def test_emulated_multi_launch_paths():
     _run(CAPPED, {"SYCLFFT_HIP_MAX_TILE": "1024"})
 
