@@ -264,6 +264,7 @@ struct plan
     void *scratch{};
     hipEvent_t last{};
     bool have_last{};
+    hipStream_t last_stream{}; // stream the last event was recorded on
     std::mutex mu;
     const sfh_kernel_variant *variants{};
     int nvariants{};
@@ -1554,7 +1555,11 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     if (p->scratch_elems) {
         hip_check(hipMalloc(&p->scratch, p->scratch_elems * p->esize), "hipMalloc(scratch)");
     }
-    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming), "hipEventCreate");
+    // The serialization event only orders device work between streams and gates plan
+    // destruction, so it needs no system-scope fence: recording it must not write back
+    // the caches after every execution (the caller's own events do that where needed).
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming | hipEventDisableSystemFence),
+              "hipEventCreate");
     auto ptr = [&](int64_t off) -> const void * {
         return off < 0 ? nullptr : static_cast<const unsigned char *>(p->tables) + off;
     };
@@ -1718,8 +1723,16 @@ static std::string describe_plan(const plan &p);
 static tuning autotune(const syclfft_hip_plan_config_v1 *cfg, int dev)
 {
     std::map<std::string, double> seen; // launch sequence -> ms
+    // split-precision plans: a candidate that runs fewer launches on MFMA than the default
+    // plan (e.g. a larger variant without radix-16 passes) is not that precision mode
+    int base_split = -1;
     auto trial = [&](const tuning &t) -> double {
         std::unique_ptr<plan, void (*)(plan *)> q(build_plan(cfg, dev, t), release);
+        if (base_split < 0) {
+            base_split = q->split_launches;
+        } else if (q->split_launches < base_split) {
+            return -1.0;
+        }
         const std::string d = describe_plan(*q);
         const auto it = seen.find(d);
         if (it != seen.end()) {
@@ -1908,7 +1921,9 @@ static void execute_plan(
             hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
         }
     }
-    if (p.have_last && !capturing) {
+    // the previous execution was enqueued on this same stream: stream order already
+    // serializes them
+    if (p.have_last && !capturing && stream != p.last_stream) {
         hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
     }
     if (p.copy_only) {
@@ -1929,6 +1944,7 @@ static void execute_plan(
     if (!capturing) {
         hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
         p.have_last = true;
+        p.last_stream = stream;
     }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
@@ -2077,7 +2093,7 @@ static void execute_units(
         }
     }
     const bool serialize = (ordered || p.scratch != nullptr) && !capturing;
-    if (serialize && p.have_last) {
+    if (serialize && p.have_last && stream != p.last_stream) {
         hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
     }
     if (pos) {
@@ -2119,6 +2135,7 @@ static void execute_units(
     if (serialize) {
         hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
         p.have_last = true;
+        p.last_stream = stream;
     }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");

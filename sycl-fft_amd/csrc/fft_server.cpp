@@ -12,9 +12,10 @@
 //                 returns their footprint and wakes the dispatcher and waiters
 //
 // Plans stay resident between requests (twiddle tables, chirp spectra, scratch) and are
-// evicted least recently used when a new plan would not fit the budget.  Requests of one
-// shape admitted while earlier launches run go to other streams and other plan instances,
-// so their work overlaps on the device.  No reference counterpart: the reference's only
+// evicted least recently used when a new plan would not fit the budget.  A plan is created
+// only when no resident plan of the shape can hold the oldest waiting request (creation
+// costs far more than a launch); requests of different shapes admitted while earlier
+// launches run go to the least loaded stream, so their work overlaps on the device.  No reference counterpart: the reference's only
 // concurrency model is one plan per concurrent stream of work (README.md:45-47).
 #include <hip/hip_runtime.h>
 
@@ -387,14 +388,33 @@ private:
                 bytes += q->bytes;
             }
         }
-        // an idle resident plan of this shape large enough for the group; with one instance
-        // per stream already resident, the least busy of them (its executions serialize);
-        // else a new one
-        plan_entry *pe = nullptr, *busy = nullptr;
-        size_t instances = 0;
+        // Plan creation (table generation and upload, allocation) costs far more than a
+        // launch, so a group larger than every resident plan of its shape is trimmed to the
+        // largest one that still holds the oldest request; a new plan is made only when none
+        // does.
         auto &same = by_key_[head->key];
+        uint64_t largest = 0;
         for (plan_entry *e : same) {
-            ++instances;
+            largest = std::max(largest, e->capacity);
+        }
+        if (largest >= head->r.batch_count && units > largest) {
+            std::vector<std::shared_ptr<request>> kept{head};
+            units = head->r.batch_count;
+            bytes = head->bytes;
+            for (size_t k = 1; k < group.size(); ++k) {
+                if (units + group[k]->r.batch_count <= largest) {
+                    units += group[k]->r.batch_count;
+                    bytes += group[k]->bytes;
+                    kept.push_back(group[k]);
+                }
+            }
+            group.swap(kept);
+        }
+        // an idle resident plan of this shape large enough for the group, else the least busy
+        // of them (its executions serialize; waiting requests of the shape merge into the next
+        // launch instead), else a new one
+        plan_entry *pe = nullptr, *busy = nullptr;
+        for (plan_entry *e : same) {
             if (e->capacity < units) {
                 continue;
             }
@@ -405,7 +425,7 @@ private:
                 busy = e;
             }
         }
-        if (!pe && busy && instances >= streams_.size()) {
+        if (!pe && busy) {
             pe = busy;
         }
         uint64_t capacity = 0, plan_bytes = 0;

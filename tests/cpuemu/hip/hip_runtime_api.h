@@ -49,6 +49,7 @@ struct hipDeviceProp_t
     int multiProcessorCount;
 };
 #define hipEventDisableTiming 2u
+#define hipEventDisableSystemFence 0x20000000
 enum hipFuncAttribute
 {
     hipFuncAttributeMaxDynamicSharedMemorySize = 8

@@ -125,3 +125,22 @@ def test_split_precision_is_deterministic_and_matches_valu_closely():
     assert np.array_equal(a, b)
     ref = oracle_fftn(x.astype(np.complex128), [n], batch, -1, "none")
     assert rel_l2(a, v) <= rel_l2(a, ref) + rel_l2(v, ref)
+
+
+@pytest.mark.parametrize("precision", ["split_fp16", "split_bf16"])
+def test_split_precision_measure_mode_keeps_mfma(precision):
+    """planning_mode::measure may pick another tile shape or variant, but never one that
+    drops the MFMA passes (e.g. a larger variant without radix-16 passes)."""
+    n, batch = 16384, 256
+    say(f"case measure-mode split plan n={n} batch={batch} precision={precision}")
+    p = sh.Plan([n], batch, sh.FORWARD, precision=precision, measure=True)
+    desc = p.describe()
+    x = uniform(n * batch, 20260700).astype(np.complex64)
+    xd = torch.from_numpy(x).to("cuda")
+    out = torch.empty_like(xd)
+    p.execute(xd, out)
+    torch.cuda.synchronize()
+    p.close()
+    assert "split=" in desc, desc
+    ref = oracle_fftn(x[: 2 * n].astype(np.complex128), [n], 2, -1, "none")
+    assert rel_l2(out[: 2 * n].cpu().numpy(), ref) <= TOL[precision]
