@@ -301,6 +301,60 @@ def precision_bench(sh, torch, dev, lib, reps=20):
     return out
 
 
+def queue_bench(sh, torch, dev, nplans=256, reps=20, seed=20260007):
+    """Work-queue execution: `nplans` small plans of mixed lengths (7-smooth 16..4096), batch
+    counts (log-uniform 1..64), precisions (80% f32), directions and normalizations.  One
+    round executes every plan once: plan by plan on one stream, or with one
+    syclfft_hip_execute_queue call (one persistent launch per kernel variant)."""
+    g = np.random.default_rng(seed)
+    lengths = [n for n in range(16, 4097) if smooth7(n)]
+    norms = ["none", "forward", "backward", "orthogonal"]
+    plans, xs, ys, fl = [], [], [], 0.0
+    for _ in range(nplans):
+        n = lengths[int(g.integers(len(lengths)))]
+        b = int(np.exp(g.uniform(0, np.log(64))))
+        dbl = bool(g.random() < 0.2)
+        p = sh.Plan([n], b, int(g.choice([-1, 1])), double=dbl, normalization=norms[int(g.integers(4))])
+        x = torch.empty(n * b, dtype=torch.complex128 if dbl else torch.complex64, device=dev)
+        x.real.uniform_(-1, 1)
+        x.imag.uniform_(-1, 1)
+        plans.append(p)
+        xs.append(x)
+        ys.append(torch.empty_like(x))
+        fl += flops(n, b)
+    stream = torch.cuda.current_stream(dev)
+
+    def one_by_one():
+        for p, x, y in zip(plans, xs, ys):
+            p.execute(x, y)
+
+    def queued():
+        sh.execute_queue(plans, xs, ys)
+
+    out = {}
+    for name, fn in (("plan_by_plan", one_by_one), ("work_queue", queued)):
+        for _ in range(2):
+            fn()
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b_.record(stream)
+        torch.cuda.synchronize()
+        host_ms = (time.perf_counter() - t0) * 1e3 / reps
+        ms = a.elapsed_time(b_) / reps
+        out[name] = {"ms_per_round": round(ms, 4), "host_ms_per_round": round(host_ms, 4),
+                     "gflops": round(fl / (ms * 1e-3) / 1e9, 1)}
+    out["speedup"] = round(out["plan_by_plan"]["ms_per_round"] / out["work_queue"]["ms_per_round"], 3)
+    out["sample"] = (f"{nplans} plans, seed {seed}: lengths 7-smooth 16..4096, batch log-uniform 1..64, "
+                     f"80% f32 / 20% f64, random direction and normalization, out of place")
+    for p in plans:
+        p.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,6 +555,11 @@ def main():
     if precision:
         log(f"precision: {precision}")
 
+    # ---------------- work-queue execution of many small plans
+    work_queue = None if args.no_serve else queue_bench(sh, torch, dev)
+    if work_queue:
+        log(f"work queue: {work_queue}")
+
     # ---------------- serving loop (continuous batching over resident plans)
     serving = None
     if not args.no_serve:
@@ -549,6 +608,7 @@ def main():
         "fp32_hbm_sweep": sweep,
         "multid": multid,
         "precision": precision,
+        "work_queue": work_queue,
         "serving": serving,
     }
     print(json.dumps(line))

@@ -156,6 +156,22 @@ SYCLFFT_HIP_API int syclfft_hip_plan_execute_many(
     void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *const *deps,
     size_t ndeps, void *done, char *err, size_t cap);
 
+/* Enqueue `count` executions of possibly different plans on `stream`: execution k runs
+ * plans[k] over its whole batch from ins[k] into outs[k], with the checks and ordering of
+ * syclfft_hip_plan_execute; `done` (optional) is recorded when all are complete.  Plans
+ * whose execution is one plain tile launch (7-smooth lengths that fit one tile) and share
+ * a kernel variant run together in ONE persistent launch: its workgroups take tiles from a
+ * device work queue, so transforms of uneven lengths, batch counts, directions and
+ * normalizations share a dispatch.  Other plans execute first, as by
+ * syclfft_hip_plan_execute.  All plans must belong to one device.  The work list is
+ * uploaded from host staging, so this call cannot be captured into a HIP graph
+ * (INVALID_STATE).  No reference counterpart: the reference runs one plan per execute
+ * (src/sycl/sycl.cpp:615-776); this is the batching entry for callers that hold many small
+ * plans, such as a request loop. */
+SYCLFFT_HIP_API int syclfft_hip_execute_queue(
+    void *const *plans, const void *const *ins, void *const *outs, size_t count, void *stream, void *done,
+    char *err, size_t cap);
+
 /* Human-readable kernel sequence of a plan (one line per launch) for tracing and tests;
  * returns the full length (like snprintf). No reference counterpart. */
 SYCLFFT_HIP_API size_t syclfft_hip_plan_describe(const void *plan, char *buf, size_t cap);

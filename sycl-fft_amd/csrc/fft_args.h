@@ -134,3 +134,23 @@ struct sfh_tile_args
     double out_sx, out_sy; // store epilogue: re * out_sx, im * out_sy
     sfh_pass pass[SFH_MAX_PASSES];
 };
+
+// Work-queue launches: one persistent launch runs the tiles of several single-launch plans
+// of one kernel variant (plain fusion mode).  Item k is tile `tile` of the launch whose
+// arguments are args[arg]; each workgroup takes the next item from *counter until none is
+// left, so plans of different lengths, batch counts and directions share one dispatch.
+struct sfh_queue_item
+{
+    uint32_t arg;
+    uint32_t tile;
+};
+
+struct sfh_queue_args
+{
+    const struct sfh_tile_args *args;
+    const struct sfh_queue_item *items;
+    uint32_t *counter; // 0 when the launch starts (written by the upload before it)
+    uint32_t nitems;
+    uint32_t lds_next; // byte offset in dynamic LDS of the item-index slot (after the largest tile)
+};
+

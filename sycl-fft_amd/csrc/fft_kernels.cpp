@@ -120,7 +120,8 @@ extern "C" int sfh_launch_tile(
     }
     // AQL dispatches carry the grid size in work-items as a 32-bit value
     if (grid == 0 || grid > args->ntiles || static_cast<uint64_t>(grid) * variant->nt > 0xffffffffull
-        || mode_index < 0 || mode_index >= SFH_NUM_MODES || !variant->fn[mode_index]) {
+        || mode_index < 0 || mode_index >= SFH_NUM_MODES || mode_index == SFH_MI_QUEUE
+        || !variant->fn[mode_index]) {
         return static_cast<int>(hipErrorInvalidConfiguration);
     }
     void *kernel_args[] = {const_cast<sfh_tile_args *>(args)};
@@ -128,4 +129,21 @@ extern "C" int sfh_launch_tile(
         variant->fn[mode_index], dim3(grid), dim3(static_cast<uint32_t>(variant->nt)), kernel_args,
         args->lds_bytes, stream);
     return static_cast<int>(st);
+}
+
+extern "C" int sfh_launch_queue(
+    const sfh_kernel_variant *variant, const sfh_queue_args *args, uint32_t grid, uint32_t lds_bytes,
+    hipStream_t stream)
+{
+    if (args->nitems == 0) {
+        return 0;
+    }
+    if (grid == 0 || grid > args->nitems || !variant->fn[SFH_MI_QUEUE] || lds_bytes > 163840
+        || args->lds_next + sizeof(uint32_t) > lds_bytes) {
+        return static_cast<int>(hipErrorInvalidConfiguration);
+    }
+    void *kernel_args[] = {const_cast<sfh_queue_args *>(args)};
+    return static_cast<int>(hipLaunchKernel(variant->fn[SFH_MI_QUEUE], dim3(grid),
+                                            dim3(static_cast<uint32_t>(variant->nt)), kernel_args, lds_bytes,
+                                            stream));
 }

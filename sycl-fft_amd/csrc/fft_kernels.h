@@ -20,7 +20,8 @@ enum sfh_mode_index
     SFH_MI_T4_BF16 = 9,   // SFH_MI_T4, same
     SFH_MI_PLAIN_F16 = 10,// SFH_MI_PLAIN with radix-16 passes on MFMA, scaled split FP16
     SFH_MI_T4_F16 = 11,   // SFH_MI_T4, same
-    SFH_NUM_MODES = 12
+    SFH_MI_QUEUE = 12,    // work-queue form of SFH_MI_PLAIN (sfh_queue_args, sfh_launch_queue)
+    SFH_NUM_MODES = 13
 };
 
 struct sfh_kernel_variant
@@ -41,5 +42,9 @@ extern "C" int sfh_launch_tile(
     hipStream_t stream);
 // workgroups of one entry resident per CU at the given dynamic LDS (0 if unknown)
 extern "C" int sfh_resident_groups(const sfh_kernel_variant *variant, int mode_index, uint32_t lds_bytes);
+// one work-queue launch (SFH_MI_QUEUE entry) of `grid` workgroups with `lds_bytes` of dynamic LDS
+extern "C" int sfh_launch_queue(
+    const sfh_kernel_variant *variant, const sfh_queue_args *args, uint32_t grid, uint32_t lds_bytes,
+    hipStream_t stream);
 extern "C" int sfh_launch_copy_scale(
     const void *in, void *out, uint64_t count, int is_double, double scale, hipStream_t stream);

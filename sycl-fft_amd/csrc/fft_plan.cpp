@@ -17,6 +17,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -246,6 +247,19 @@ struct tuning
     }
 };
 
+// One staging slot of the work-queue ring (see execute_queue): device block, pinned host
+// copy, and the event after which the slot's last launch has completed; `gen` counts its
+// launches, so a plan can tell whether the slot still holds the launch it was part of.
+struct queue_slot
+{
+    void *dev{};
+    void *host{};
+    size_t bytes{};
+    hipEvent_t free{};
+    bool used{};
+    std::atomic<uint64_t> gen{0};
+};
+
 struct plan
 {
     syclfft_hip_plan_config_v1 cfg{};
@@ -264,7 +278,11 @@ struct plan
     void *scratch{};
     hipEvent_t last{};
     bool have_last{};
-    hipStream_t last_stream{}; // stream the last event was recorded on
+    hipStream_t last_stream{}; // stream of the last execution
+    // the last execution was part of a work-queue launch: its completion is that slot's event
+    // while the slot's generation is still last_qgen (the slot is reused only after it completes)
+    queue_slot *last_queue{};
+    uint64_t last_qgen{};
     std::mutex mu;
     const sfh_kernel_variant *variants{};
     int nvariants{};
@@ -282,6 +300,47 @@ struct plan
     bool unit_used[k_unit_slots]{};
     int unit_next{};
 };
+
+// Completion event of the plan's last execution (null: known complete).
+static hipEvent_t last_event(const plan &p)
+{
+    if (!p.have_last) {
+        return nullptr;
+    }
+    if (p.last_queue) {
+        return p.last_queue->gen.load() == p.last_qgen ? p.last_queue->free : nullptr;
+    }
+    return p.last;
+}
+
+// Order `stream` after the plan's last execution (nothing to do on the same stream).
+static void wait_last(const plan &p, hipStream_t stream)
+{
+    if (!p.have_last || stream == p.last_stream) {
+        return;
+    }
+    if (hipEvent_t e = last_event(p)) {
+        if (hipStreamWaitEvent(stream, e, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::runtime_error("hipStreamWaitEvent(last) failed");
+        }
+    }
+}
+
+// Block the host until the plan's last execution has completed.
+static void sync_last(const plan &p)
+{
+    if (hipEvent_t e = last_event(p)) {
+        (void)hipEventSynchronize(e);
+    }
+}
+
+static void set_last(plan &p, hipStream_t stream)
+{
+    p.have_last = true;
+    p.last_stream = stream;
+    p.last_queue = nullptr;
+}
 
 static bool variant_allowed(const plan &p, int i)
 {
@@ -1403,9 +1462,7 @@ static void release(plan *p)
     if (!p) {
         return;
     }
-    if (p->have_last) {
-        (void)hipEventSynchronize(p->last);
-    }
+    sync_last(*p);
     if (p->last) {
         (void)hipEventDestroy(p->last);
     }
@@ -1696,9 +1753,7 @@ static double time_plan(plan &p)
         }
         (void)hipStreamSynchronize(st);
     }
-    if (p.have_last) {
-        (void)hipEventSynchronize(p.last);
-    }
+    sync_last(p);
     if (e0) {
         (void)hipEventDestroy(e0);
     }
@@ -1891,9 +1946,8 @@ static void validate_pointer(const plan &p, const void *ptr, uint64_t bytes, con
     }
 }
 
-static void execute_plan(
-    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
-    hipEvent_t done)
+// The reference's execute() argument checks (src/sycl/sycl.cpp:621-635) plus device-range checks.
+static void check_buffers(const plan &p, const void *in, const void *out)
 {
     const uint64_t bytes = p.total * p.esize;
     validate_pointer(p, in, bytes, "input");
@@ -1913,6 +1967,13 @@ static void execute_plan(
                 SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
         }
     }
+}
+
+static void execute_plan(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
+    hipEvent_t done)
+{
+    check_buffers(p, in, out);
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
     const bool capturing = stream_capturing(stream);
@@ -1923,8 +1984,8 @@ static void execute_plan(
     }
     // the previous execution was enqueued on this same stream: stream order already
     // serializes them
-    if (p.have_last && !capturing && stream != p.last_stream) {
-        hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
+    if (!capturing) {
+        wait_last(p, stream);
     }
     if (p.copy_only) {
         hip_check(
@@ -1943,8 +2004,7 @@ static void execute_plan(
     }
     if (!capturing) {
         hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
-        p.have_last = true;
-        p.last_stream = stream;
+        set_last(p, stream);
     }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
@@ -2093,8 +2153,8 @@ static void execute_units(
         }
     }
     const bool serialize = (ordered || p.scratch != nullptr) && !capturing;
-    if (serialize && p.have_last && stream != p.last_stream) {
-        hip_check(hipStreamWaitEvent(stream, p.last, 0), "hipStreamWaitEvent(last)");
+    if (serialize) {
+        wait_last(p, stream);
     }
     if (pos) {
         hip_check(hipMemcpyAsync(dev, host, pos * sizeof(void *), hipMemcpyHostToDevice, stream),
@@ -2134,8 +2194,233 @@ static void execute_units(
     }
     if (serialize) {
         hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
-        p.have_last = true;
-        p.last_stream = stream;
+        set_last(p, stream);
+    }
+    if (done) {
+        hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Work-queue execution: executions of several plans in one persistent launch per kernel
+// variant.  Plans whose execution is one plain-mode launch are grouped by variant; each
+// group's launch arguments (one per execution) and work items (one per tile) are uploaded
+// from a pinned staging ring and run by the variant's tile_fft_queue entry, whose
+// workgroups take items from a device counter.  Other plans execute as usual first, on
+// the same stream.
+// ------------------------------------------------------------------------------------
+
+struct queue_ctx
+{
+    static constexpr int k_slots = 8;
+    std::mutex mu;
+    queue_slot slot[k_slots];
+    int next{};
+};
+
+// One context per device for the life of the process (executions of any plan may still read
+// its slots until they complete).
+static queue_ctx &queue_context(int dev)
+{
+    static std::mutex m;
+    static std::map<int, queue_ctx *> ctxs;
+    std::lock_guard<std::mutex> l(m);
+    queue_ctx *&c = ctxs[dev];
+    if (!c) {
+        c = new queue_ctx();
+    }
+    return *c;
+}
+
+static bool queueable(const plan &p)
+{
+    // the queue launch adds a 16-byte item slot behind the largest tile
+    return !p.copy_only && p.launches.size() == 1 && p.launches[0].mode == SFH_MI_PLAIN
+           && p.variants[p.launches[0].variant].fn[SFH_MI_QUEUE] != nullptr
+           && p.launches[0].args.lds_bytes + 32u <= k_lds_limit;
+}
+
+static void execute_queue(plan *const *plans, const void *const *ins, void *const *outs, size_t count,
+                          hipStream_t stream, hipEvent_t done)
+{
+    if (count == 0) {
+        if (done) {
+            hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+        }
+        return;
+    }
+    if (!plans || !ins || !outs) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plan and buffer lists must be non-null");
+    }
+    // allocations already validated in this call: buffers carved from one allocation (the
+    // usual case for caching allocators) cost one range query for all of them
+    std::vector<std::pair<uintptr_t, uintptr_t>> known;
+    const auto check = [&](const plan &p, const void *ptr, const char *name) {
+        const uint64_t bytes = p.total * p.esize;
+        const auto a = reinterpret_cast<uintptr_t>(ptr);
+        for (const auto &r : known) {
+            if (ptr && a >= r.first && a < r.second && bytes <= r.second - a) {
+                return;
+            }
+        }
+        uintptr_t lo = 0, hi = 0;
+        validate_pointer(p, ptr, bytes, name, &lo, &hi);
+        if (lo < hi) {
+            known.emplace_back(lo, hi);
+        }
+    };
+    for (size_t k = 0; k < count; ++k) {
+        if (!plans[k]) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE, "plan handle is null");
+        }
+        const plan &p = *plans[k];
+        if (p.device != plans[0]->device) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plans of one queue execution must share a device");
+        }
+        check(p, ins[k], "input");
+        check(p, outs[k], "output");
+        const uint64_t bytes = p.total * p.esize;
+        if (p.cfg.in_place && ins[k] != outs[k]) {
+            throw sfh_error(
+                SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "In-place plan must be executed with execute(inout)");
+        }
+        if (!p.cfg.in_place) {
+            const auto a = reinterpret_cast<uintptr_t>(ins[k]), b = reinterpret_cast<uintptr_t>(outs[k]);
+            if (a == b) {
+                throw sfh_error(
+                    SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place plan requires distinct allocations");
+            }
+            if (a < b + bytes && b < a + bytes) {
+                throw sfh_error(
+                    SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
+            }
+        }
+    }
+    const int dev = plans[0]->device;
+    device_guard guard(dev);
+    if (stream_capturing(stream)) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE,
+                        "queue executions upload their work list from host staging and cannot be captured");
+    }
+    // (precision, variant) -> executions
+    std::map<std::pair<int, int>, std::vector<size_t>> groups;
+    for (size_t k = 0; k < count; ++k) {
+        const plan &p = *plans[k];
+        if (queueable(p)) {
+            groups[{p.dbl ? 1 : 0, p.launches[0].variant}].push_back(k);
+        } else {
+            execute_plan(*plans[k], ins[k], outs[k], stream, nullptr, 0, nullptr);
+        }
+    }
+    if (groups.empty()) {
+        if (done) {
+            hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+        }
+        return;
+    }
+    // each queued plan once, locked in address order (no lock-order inversion with other callers)
+    std::vector<plan *> locked;
+    for (const auto &g : groups) {
+        for (size_t k : g.second) {
+            locked.push_back(plans[k]);
+        }
+    }
+    std::sort(locked.begin(), locked.end());
+    locked.erase(std::unique(locked.begin(), locked.end()), locked.end());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(locked.size());
+    for (plan *p : locked) {
+        locks.emplace_back(p->mu);
+    }
+    for (plan *p : locked) {
+        wait_last(*p, stream);
+    }
+    queue_ctx &qc = queue_context(dev);
+    std::lock_guard<std::mutex> ql(qc.mu);
+    const int cus = sfh_device_cus();
+    for (const auto &g : groups) {
+        const std::vector<size_t> &ks = g.second;
+        const plan &p0 = *plans[ks[0]];
+        const sfh_kernel_variant &v = p0.variants[g.first.second];
+        uint64_t nitems = 0;
+        uint32_t lds = 0;
+        for (size_t k : ks) {
+            nitems += plans[k]->launches[0].args.ntiles;
+            lds = std::max(lds, plans[k]->launches[0].args.lds_bytes);
+        }
+        if (nitems > 0xffffffffull) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "queue execution has too many tiles");
+        }
+        const uint32_t lds_next = (lds + 15u) & ~15u;
+        const uint32_t lds_total = lds_next + 16u;
+        const size_t args_off = 16;
+        const size_t items_off = args_off + ks.size() * sizeof(sfh_tile_args);
+        const size_t bytes = items_off + nitems * sizeof(sfh_queue_item);
+        queue_slot &sl = qc.slot[qc.next++ % queue_ctx::k_slots];
+        if (sl.used) {
+            hip_check(hipEventSynchronize(sl.free), "hipEventSynchronize(queue slot)");
+        }
+        sl.gen.fetch_add(1); // plans of the slot's previous launch now read it as complete
+        if (sl.bytes < bytes) {
+            if (sl.dev) {
+                (void)hipFree(sl.dev);
+                (void)hipHostFree(sl.host);
+                sl.dev = sl.host = nullptr;
+                sl.bytes = 0;
+            }
+            const size_t want = std::max<size_t>(bytes + bytes / 2, 64 << 10);
+            hip_check(hipMalloc(&sl.dev, want), "hipMalloc(queue slot)");
+            hip_check(hipHostMalloc(&sl.host, want, hipHostMallocDefault), "hipHostMalloc(queue slot)");
+            sl.bytes = want;
+        }
+        if (!sl.free) {
+            hip_check(hipEventCreateWithFlags(&sl.free, hipEventDisableTiming | hipEventDisableSystemFence),
+                      "hipEventCreate(queue slot)");
+        }
+        unsigned char *h = static_cast<unsigned char *>(sl.host);
+        std::memset(h, 0, args_off); // the item counter starts at 0
+        sfh_tile_args *args = reinterpret_cast<sfh_tile_args *>(h + args_off);
+        sfh_queue_item *items = reinterpret_cast<sfh_queue_item *>(h + items_off);
+        uint64_t n = 0;
+        for (size_t i = 0; i < ks.size(); ++i) {
+            const plan &p = *plans[ks[i]];
+            const launch &l = p.launches[0];
+            void *bufs[3] = {const_cast<void *>(ins[ks[i]]), outs[ks[i]], p.scratch};
+            sfh_tile_args a = l.args;
+            a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
+            a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
+            std::memcpy(&args[i], &a, sizeof(a));
+            for (uint64_t t = 0; t < a.ntiles; ++t) {
+                items[n++] = sfh_queue_item{static_cast<uint32_t>(i), static_cast<uint32_t>(t)};
+            }
+        }
+        hip_check(hipMemcpyAsync(sl.dev, sl.host, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(queue)");
+        unsigned char *d = static_cast<unsigned char *>(sl.dev);
+        sfh_queue_args q{};
+        q.args = reinterpret_cast<const sfh_tile_args *>(d + args_off);
+        q.items = reinterpret_cast<const sfh_queue_item *>(d + items_off);
+        q.counter = reinterpret_cast<uint32_t *>(d);
+        q.nitems = static_cast<uint32_t>(nitems);
+        q.lds_next = lds_next;
+        const int per_cu = std::max(1, sfh_resident_groups(&v, SFH_MI_QUEUE, lds_total));
+        const uint64_t grid = std::min<uint64_t>(nitems, static_cast<uint64_t>(per_cu) * std::max(1, cus));
+        hip_check(static_cast<hipError_t>(sfh_launch_queue(&v, &q, static_cast<uint32_t>(grid), lds_total, stream)),
+                  "queue kernel launch");
+        trace("queue launch " + std::string(g.first.first ? "f64" : "f32") + " nt=" + std::to_string(v.nt)
+              + " e=" + std::to_string(v.e) + " executions=" + std::to_string(ks.size())
+              + " items=" + std::to_string(nitems) + " grid=" + std::to_string(grid)
+              + " lds=" + std::to_string(lds_total));
+        hip_check(hipEventRecord(sl.free, stream), "hipEventRecord(queue slot)");
+        sl.used = true;
+        // the slot's event now marks these plans' last execution (no event record per plan)
+        const uint64_t gen = sl.gen.load();
+        for (size_t k : ks) {
+            plan &p = *plans[k];
+            p.have_last = true;
+            p.last_stream = stream;
+            p.last_queue = &sl;
+            p.last_qgen = gen;
+        }
     }
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
@@ -2301,6 +2586,27 @@ extern "C" int syclfft_hip_plan_execute_many(
 // when it is submitted) and which order executions themselves (each request has its own
 // buffers); only the per-transform placement rules are still checked, and executions of a
 // plan without scratch may overlap.
+extern "C" int syclfft_hip_execute_queue(
+    void *const *plans, const void *const *ins, void *const *outs, size_t count, void *stream, void *done,
+    char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        sfh::execute_queue(reinterpret_cast<sfh::plan *const *>(plans), ins, outs, count,
+                           static_cast<hipStream_t>(stream), static_cast<hipEvent_t>(done));
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
 extern "C" __attribute__((visibility("hidden"))) int sfh_plan_execute_many_checked_buffers(
     void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *done, char *err,
     size_t cap)

@@ -1402,6 +1402,41 @@ struct occupancy
     static constexpr int WAVES = RAW < 1 ? 1 : (RAW > 8 ? 8 : RAW);
 };
 
+// One tile g of a launch: stage the twiddle tables, load, run the fused program, store.
+template <typename T, int NT, int E, int MODE>
+__device__ __forceinline__ void one_tile(const sfh_tile_args &a, cx<T> *lds, uint64_t g, uint32_t tid)
+{
+    tile_ctx<T> c;
+    if constexpr (split_prec(MODE) != 0) {
+        split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
+    }
+    cx<T> *const tab = lds + a.lds_tab;
+    const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
+    const uint32_t tab_count = a.tw_entries + a.t4_entries;
+    c.tw = tab;
+    c.t4 = tab + a.tw_entries;
+    // Twiddle tables (a few KiB) go to LDS behind the tile.  They are loaded right after the
+    // tile's pass-0 loads have been issued, so the two HBM/L2 round trips overlap instead of
+    // running one after the other.
+    const auto finish_tables = [=]() {
+        for (uint32_t e = tid; e < tab_count; e += NT) {
+            tab[e] = tab_src[e];
+        }
+        __syncthreads();
+    };
+    set_tile<T, MODE>(a, c, g, tid);
+    if (a.flags & SFH_LOAD_LDS) {
+        finish_tables();
+        load_via_lds<T, NT, E, MODE>(a, c, lds);
+        run_program<T, NT, E, MODE>(a, c, lds, IN_LDS, nullptr, [] {});
+    } else {
+        run_program<T, NT, E, MODE>(a, c, lds, IN_GLOBAL, nullptr, [] {}, finish_tables);
+    }
+    if (a.flags & SFH_STORE_LDS) {
+        store_via_lds<T, NT, E, MODE>(a, c, lds);
+    }
+}
+
 // PREF = 1: persistent workgroups with register prefetch of the next tile's pass-0 inputs
 // (launched with at most the resident number of workgroups); PREF = 0: pass 0 reads HBM
 // directly (one tile per workgroup, full occupancy).
@@ -1419,43 +1454,29 @@ void tile_fft(const sfh_tile_args a_byval)
 #endif
     SFH_DYNAMIC_LDS(sfh_smem);
     cx<T> *lds = reinterpret_cast<cx<T> *>(sfh_smem);
-
     const uint32_t tid = threadIdx.x;
+
+    if constexpr (!PREF) {
+        one_tile<T, NT, E, MODE>(a, lds, blockIdx.x, tid);
+        return;
+    }
     tile_ctx<T> c;
+    if constexpr (split_prec(MODE) != 0) {
+        split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
+    }
     cx<T> *const tab = lds + a.lds_tab;
     const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
     const uint32_t tab_count = a.tw_entries + a.t4_entries;
     c.tw = tab;
     c.t4 = tab + a.tw_entries;
-
     const uint32_t f = a.flags;
-    // Twiddle tables (a few KiB) go to LDS behind the tile.  They are loaded right after the
-    // tile's pass-0 loads have been issued, so the two HBM/L2 round trips overlap instead of
-    // running one after the other.
     const auto finish_tables = [=]() {
         for (uint32_t e = tid; e < tab_count; e += NT) {
             tab[e] = tab_src[e];
         }
         __syncthreads();
     };
-
-    if constexpr (split_prec(MODE) != 0) {
-        split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
-    }
-    if constexpr (!PREF) {
-        // one tile per workgroup
-        set_tile<T, MODE>(a, c, blockIdx.x, tid);
-        if (f & SFH_LOAD_LDS) {
-            finish_tables();
-            load_via_lds<T, NT, E, MODE>(a, c, lds);
-            run_program<T, NT, E, MODE>(a, c, lds, IN_LDS, nullptr, [] {});
-        } else {
-            run_program<T, NT, E, MODE>(a, c, lds, IN_GLOBAL, nullptr, [] {}, finish_tables);
-        }
-        if (f & SFH_STORE_LDS) {
-            store_via_lds<T, NT, E, MODE>(a, c, lds);
-        }
-    } else {
+    if constexpr (PREF) {
         // Persistent loop over tiles g = blockIdx.x, + gridDim.x, ...  The inputs of the next
         // tile's pass 0 are loaded into registers right after the current tile's pass 0, so
         // the HBM reads overlap the LDS passes (no pass waits on global memory).
@@ -1487,6 +1508,38 @@ void tile_fft(const sfh_tile_args a_byval)
             g = gn;
             c = cn;
         }
+    }
+}
+
+// Work-queue form: persistent workgroups take items (tile of some launch) from a device
+// counter until every item is taken; each item runs with its own launch's arguments, so
+// single-launch plans of any length, batch count, direction and normalization that share
+// this variant run in one dispatch.  Every workgroup leaves once the counter passes nitems.
+template <typename T, int NT, int E, int MODE>
+__global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(occupancy<T, NT, E>::WAVES)
+void tile_fft_queue(const sfh_queue_args q)
+{
+    SFH_DYNAMIC_LDS(sfh_smem);
+    cx<T> *lds = reinterpret_cast<cx<T> *>(sfh_smem);
+    uint32_t *next = reinterpret_cast<uint32_t *>(sfh_smem + q.lds_next);
+    const uint32_t tid = threadIdx.x;
+    for (;;) {
+        // the barrier also ends the previous item's last LDS reads before the next item's writes
+        if (tid == 0) {
+            *next = atomicAdd(q.counter, 1u);
+        }
+        __syncthreads();
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t k = __builtin_amdgcn_readfirstlane(*next);
+#else
+        const uint32_t k = *next;
+#endif
+        __syncthreads(); // every lane has read the slot before lane 0 takes the next item
+        if (k >= q.nitems) {
+            return;
+        }
+        const sfh_queue_item it = q.items[k];
+        one_tile<T, NT, E, MODE>(q.args[it.arg], lds, it.tile, tid);
     }
 }
 

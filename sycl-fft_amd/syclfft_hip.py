@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_plan_scratch_bytes",
     "syclfft_hip_plan_execute",
     "syclfft_hip_plan_execute_many",
+    "syclfft_hip_execute_queue",
     "syclfft_hip_plan_describe",
     "syclfft_hip_plan_launch_count",
     "syclfft_hip_server_create",
@@ -142,6 +143,10 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
         ctypes.c_size_t]
     lib.syclfft_hip_plan_execute_many.restype = ctypes.c_int
+    lib.syclfft_hip_execute_queue.argtypes = [
+        ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_execute_queue.restype = ctypes.c_int
     lib.syclfft_hip_plan_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.syclfft_hip_plan_describe.restype = ctypes.c_size_t
     lib.syclfft_hip_plan_launch_count.argtypes = [ctypes.c_void_p]
@@ -331,6 +336,32 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def execute_queue_ptr(plans: Sequence["Plan"], ins: Sequence[int], outs: Sequence[int], stream: int = 0,
+                      done: int = 0) -> None:
+    """Execute plans[k] on raw device pointers ins[k] -> outs[k], k = 0..n-1, with one
+    persistent work-queue launch per kernel variant (syclfft_hip_execute_queue)."""
+    if not (len(plans) == len(ins) == len(outs)):
+        raise ValueError("plans, ins and outs must have the same length")
+    n = len(plans)
+    lib = load()
+    pa = (ctypes.c_void_p * max(1, n))(*[p.handle.value for p in plans])
+    ia = (ctypes.c_void_p * max(1, n))(*[int(v) for v in ins])
+    oa = (ctypes.c_void_p * max(1, n))(*[int(v) for v in outs])
+    err = ctypes.create_string_buffer(1024)
+    st = lib.syclfft_hip_execute_queue(pa, ia, oa, n, ctypes.c_void_p(stream), ctypes.c_void_p(done), err, 1024)
+    if st != 0:
+        _raise(st, err)
+
+
+def execute_queue(plans: Sequence["Plan"], xs, ys=None, stream=None) -> None:
+    """execute_queue_ptr over torch tensors (ys defaults to xs: in-place plans)."""
+    import torch
+
+    ys = xs if ys is None else ys
+    s = stream if stream is not None else torch.cuda.current_stream(xs[0].device)
+    execute_queue_ptr(plans, [x.data_ptr() for x in xs], [y.data_ptr() for y in ys], s.cuda_stream)
 
 
 def provider_table(lib: Optional[ctypes.CDLL] = None) -> ProviderTable:

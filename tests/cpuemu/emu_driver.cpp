@@ -192,8 +192,133 @@ static bool run_case(const std::string &spec)
     return ok;
 }
 
+// EMU_QUEUE=1: every case's plan is created first and all executions go through one
+// syclfft_hip_execute_queue call (plain single-launch plans share work-queue launches).
+struct queued
+{
+    std::string spec;
+    std::vector<uint64_t> lens;
+    uint64_t batch{}, total{};
+    int dir{}, norm{}, inplace{};
+    bool dbl{};
+    void *plan{}, *din{}, *dout{};
+    std::vector<double> x;
+};
+
+static bool run_queue(int argc, char **argv)
+{
+    std::vector<queued> qs;
+    char err[1024];
+    for (int i = 1; i < argc; ++i) {
+        queued q;
+        q.spec = argv[i];
+        char prec = 0;
+        char dims[128] = {0};
+        unsigned long long batch = 1;
+        if (std::sscanf(argv[i], "%c:%127[0-9x]:%llu:%d:%d:%d", &prec, dims, &batch, &q.dir, &q.norm,
+                        &q.inplace) != 6) {
+            std::printf("FAIL %s bad case spec\n", argv[i]);
+            return false;
+        }
+        for (char *tok = std::strtok(dims, "x"); tok; tok = std::strtok(nullptr, "x")) {
+            q.lens.push_back(std::strtoull(tok, nullptr, 10));
+        }
+        q.dbl = prec == 'd';
+        q.batch = batch;
+        syclfft_hip_plan_config_v1 cfg{};
+        cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
+        cfg.scalar = q.dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
+        cfg.direction = q.dir;
+        cfg.rank = static_cast<uint32_t>(q.lens.size());
+        for (size_t k = 0; k < q.lens.size(); ++k) {
+            cfg.lengths[k] = q.lens[k];
+        }
+        cfg.batch_count = batch;
+        cfg.in_place = static_cast<uint32_t>(q.inplace);
+        cfg.normalization = static_cast<uint32_t>(q.norm);
+        cfg.device = -1;
+        if (syclfft_hip_plan_create_ex(&cfg, &q.plan, err, sizeof(err)) != 0) {
+            std::printf("FAIL %s create: %s\n", argv[i], err);
+            return false;
+        }
+        q.total = batch;
+        for (auto n : q.lens) {
+            q.total *= n;
+        }
+        q.x.resize(2 * q.total);
+        std::mt19937_64 rng(20260000 + q.total + i);
+        std::uniform_real_distribution<double> u(-1.0, 1.0);
+        for (auto &v : q.x) {
+            v = u(rng);
+        }
+        const size_t es = q.dbl ? 16 : 8;
+        hipMalloc(&q.din, q.total * es);
+        q.dout = q.din;
+        if (!q.inplace) {
+            hipMalloc(&q.dout, q.total * es);
+        }
+        if (q.dbl) {
+            std::memcpy(q.din, q.x.data(), q.total * 16);
+        } else {
+            float *f = static_cast<float *>(q.din);
+            for (uint64_t k = 0; k < 2 * q.total; ++k) {
+                f[k] = static_cast<float>(q.x[k]);
+                q.x[k] = f[k];
+            }
+        }
+        qs.push_back(std::move(q));
+    }
+    std::vector<void *> plans, outs;
+    std::vector<const void *> ins;
+    for (auto &q : qs) {
+        plans.push_back(q.plan);
+        ins.push_back(q.din);
+        outs.push_back(q.dout);
+    }
+    if (syclfft_hip_execute_queue(plans.data(), ins.data(), outs.data(), qs.size(), nullptr, nullptr, err,
+                                  sizeof(err)) != 0) {
+        std::printf("FAIL queue execute: %s\n", err);
+        return false;
+    }
+    bool all = true;
+    for (auto &q : qs) {
+        std::vector<double> got(2 * q.total), ref(2 * q.total);
+        if (q.dbl) {
+            std::memcpy(got.data(), q.dout, q.total * 16);
+        } else {
+            const float *f = static_cast<const float *>(q.dout);
+            for (uint64_t k = 0; k < 2 * q.total; ++k) {
+                got[k] = f[k];
+            }
+        }
+        oracle_fftn_ld(static_cast<int>(q.lens.size()), q.lens.data(), q.batch, q.x.data(), ref.data(), q.dir,
+                       q.norm);
+        double num = 0, den = 0;
+        for (uint64_t k = 0; k < 2 * q.total; ++k) {
+            num += (got[k] - ref[k]) * (got[k] - ref[k]);
+            den += ref[k] * ref[k];
+        }
+        const double rel = std::sqrt(num / (den > 0 ? den : 1));
+        const double u_ = q.dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+        const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(q.total / q.batch))) * u_;
+        const bool ok = rel <= tol && std::isfinite(rel);
+        std::printf("%s %s relL2=%.3e tol=%.1e (queue)\n", ok ? "ok" : "FAIL", q.spec.c_str(), rel, tol);
+        all = all && ok;
+        syclfft_hip_plan_destroy(q.plan);
+        hipFree(q.din);
+        if (!q.inplace) {
+            hipFree(q.dout);
+        }
+    }
+    std::fflush(stdout);
+    return all;
+}
+
 int main(int argc, char **argv)
 {
+    if (std::getenv("EMU_QUEUE")) {
+        return run_queue(argc, argv) ? 0 : 1;
+    }
     bool all = true;
     for (int i = 1; i < argc; ++i) {
         all = run_case(argv[i]) && all;

@@ -165,8 +165,47 @@ void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body)
     g_bar = nullptr;
 }
 
+namespace
+{
+std::mutex g_qmu;
+std::map<const void *, void (*)(void **)> g_queue_kernels;
+} // namespace
+
+void emu_register_queue_kernel(const void *fn, void (*call)(void **args))
+{
+    std::lock_guard<std::mutex> l(g_qmu);
+    g_queue_kernels[fn] = call;
+}
+
+uint32_t emu_atomic_add_u32(uint32_t *p, uint32_t v)
+{
+    std::lock_guard<std::mutex> l(g_qmu);
+    const uint32_t old = *p;
+    *p = old + v;
+    return old;
+}
+
 hipError_t hipLaunchKernel(const void *fn, dim3 grid, dim3 block, void **args, size_t shmem, hipStream_t)
 {
+    void (*queue_call)(void **) = nullptr;
+    {
+        std::lock_guard<std::mutex> l(g_qmu);
+        const auto it = g_queue_kernels.find(fn);
+        if (it != g_queue_kernels.end()) {
+            queue_call = it->second;
+        }
+    }
+    if (queue_call) {
+        if (block.x == 0 || block.x > 1024 || grid.x == 0 || shmem > 163840) {
+            return hipErrorInvalidConfiguration;
+        }
+        unsigned char *lds = static_cast<unsigned char *>(std::malloc(shmem ? shmem : 1));
+        g_dyn_lds = lds;
+        emu_launch(grid, block, [&]() { queue_call(args); });
+        g_dyn_lds = nullptr;
+        std::free(lds);
+        return hipSuccess;
+    }
     // every kernel launched through this entry takes one sfh_tile_args by value
     auto kernel = reinterpret_cast<void (*)(sfh_tile_args)>(const_cast<void *>(fn));
     const sfh_tile_args a = *static_cast<const sfh_tile_args *>(args[0]);

@@ -13,6 +13,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define atomicAdd(p, v) emu_atomic_add_u32((p), (v))
 #define __shared__ static
 
 extern thread_local dim3 threadIdx;
@@ -21,6 +22,7 @@ extern thread_local dim3 blockDim;
 extern thread_local dim3 gridDim;
 
 void emu_syncthreads();
+uint32_t emu_atomic_add_u32(uint32_t *p, uint32_t v);
 // dynamic LDS of the running launch (sized by the launch's sharedMem argument)
 unsigned char *emu_dynamic_lds();
 #define __syncthreads() emu_syncthreads()
@@ -46,6 +48,10 @@ void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body);
 
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                               \
     emu_launch(dim3(grid), dim3(block), [&]() { kernel(__VA_ARGS__); })
+
+// work-queue kernels take one sfh_queue_args argument: registered so that hipLaunchKernel
+// calls them through `call` (every other entry takes one sfh_tile_args)
+void emu_register_queue_kernel(const void *fn, void (*call)(void **args));
 
 // tile kernels are launched through hipLaunchKernel with one sfh_tile_args argument
 void emu_register_tile_kernel(const void *fn, void (*call)(const void *fn, void **args));

@@ -138,3 +138,14 @@ def test_emulated_bluestein_two_workgroups():
     _run(HALVES, {"SYCLFFT_HIP_BLUE_HALVES": "1"})
     _run(["f:10007:3:-1:3:0", "d:1009:3:1:1:1"], {"SYCLFFT_HIP_BLUE_HALVES": "1", "EMU_MANY": "1"})
     _run(["f:1009:5:-1:3:0"], {"SYCLFFT_HIP_BLUE_HALVES": "1", "EMU_MANY": "2"})
+
+
+def test_emulated_work_queue_execution():
+    """One syclfft_hip_execute_queue call over plans of mixed lengths, batch counts, precisions,
+    directions, normalizations and placements: single-launch plain plans share persistent
+    work-queue launches per kernel variant (the emulated device counter hands out their
+    tiles), the others (Bluestein, four-step, multi-D) execute first on the same stream."""
+    _run(["f:256:5:-1:0:0", "f:240:3:1:3:0", "f:4096:2:-1:1:1", "d:2520:2:1:3:0", "f:16:33:-1:0:0",
+          "f:10007:2:-1:0:0", "d:64:7:-1:0:1", "f:512:4:1:0:0", "f:1000:3:-1:0:0", "f:32x32:2:1:2:0",
+          "f:65536:1:-1:0:0"], {"EMU_QUEUE": "1"})
+
