@@ -2212,7 +2212,7 @@ static void execute_units(
 
 struct queue_ctx
 {
-    static constexpr int k_slots = 8;
+    static constexpr int k_slots = 32;
     std::mutex mu;
     queue_slot slot[k_slots];
     int next{};
@@ -2240,8 +2240,45 @@ static bool queueable(const plan &p)
            && p.launches[0].args.lds_bytes + 32u <= k_lds_limit;
 }
 
+// Whether `count` transforms of the plan (count < batch: the first `count` of a contiguous
+// batch) can join a queue launch: the launch's transform dimension must be annotated as for
+// pointer-array execution, so it can be cut to `count`.
+static bool queueable_for(const plan &p, uint64_t count)
+{
+    if (!queueable(p) || count == 0 || count > p.cfg.batch_count) {
+        return false;
+    }
+    if (count == p.cfg.batch_count) {
+        return true;
+    }
+    const launch &l = p.launches[0];
+    return l.unit_dim != 0 && l.unit_extent % p.cfg.batch_count == 0 && l.src_off == 0 && l.dst_off == 0;
+}
+
+// Cut a launch's transform dimension to `count` of the plan's B transforms.
+static void cut_to_count(const launch &l, sfh_tile_args &a, uint64_t count, uint64_t B)
+{
+    const uint64_t ext = count * (l.unit_extent / B);
+    const uint64_t nbhi = a.ntiles / (a.nwt * a.nblo);
+    if (l.unit_dim == SFH_UNIT_COL) {
+        a.wcount = ext;
+        a.nwt = (ext + a.W - 1) / a.W;
+        a.ntiles = a.nwt * a.nblo * nbhi;
+    } else if (l.unit_dim == SFH_UNIT_BLO) {
+        a.nblo = ext;
+        a.ntiles = a.nwt * ext * nbhi;
+    } else {
+        a.ntiles = a.nwt * a.nblo * ext;
+    }
+}
+
+// counts (optional): entry k runs the first counts[k] transforms of plans[k] (contiguous from
+// ins[k] / outs[k]); validate = false: buffers and placement were checked by the caller, and
+// every entry must be queueable; ordered = false: no wait for the plans' previous executions
+// on other streams (single-launch plans own no scratch; the serving loop's use)
 static void execute_queue(plan *const *plans, const void *const *ins, void *const *outs, size_t count,
-                          hipStream_t stream, hipEvent_t done)
+                          hipStream_t stream, hipEvent_t done, const uint64_t *counts = nullptr,
+                          bool validate = true, bool ordered = true)
 {
     if (count == 0) {
         if (done) {
@@ -2277,6 +2314,12 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         if (p.device != plans[0]->device) {
             throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plans of one queue execution must share a device");
         }
+        if (counts && !queueable_for(p, counts[k])) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "entry cannot join a queue launch");
+        }
+        if (!validate) {
+            continue;
+        }
         check(p, ins[k], "input");
         check(p, outs[k], "output");
         const uint64_t bytes = p.total * p.esize;
@@ -2306,7 +2349,7 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
     std::map<std::pair<int, int>, std::vector<size_t>> groups;
     for (size_t k = 0; k < count; ++k) {
         const plan &p = *plans[k];
-        if (queueable(p)) {
+        if (counts || queueable(p)) {
             groups[{p.dbl ? 1 : 0, p.launches[0].variant}].push_back(k);
         } else {
             execute_plan(*plans[k], ins[k], outs[k], stream, nullptr, 0, nullptr);
@@ -2333,7 +2376,9 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         locks.emplace_back(p->mu);
     }
     for (plan *p : locked) {
-        wait_last(*p, stream);
+        if (ordered) {
+            wait_last(*p, stream);
+        }
     }
     queue_ctx &qc = queue_context(dev);
     std::lock_guard<std::mutex> ql(qc.mu);
@@ -2342,11 +2387,23 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         const std::vector<size_t> &ks = g.second;
         const plan &p0 = *plans[ks[0]];
         const sfh_kernel_variant &v = p0.variants[g.first.second];
+        // each entry's launch arguments: its buffers, its transform count
+        std::vector<sfh_tile_args> eargs(ks.size());
         uint64_t nitems = 0;
         uint32_t lds = 0;
-        for (size_t k : ks) {
-            nitems += plans[k]->launches[0].args.ntiles;
-            lds = std::max(lds, plans[k]->launches[0].args.lds_bytes);
+        for (size_t i = 0; i < ks.size(); ++i) {
+            const plan &p = *plans[ks[i]];
+            const launch &l = p.launches[0];
+            void *bufs[3] = {const_cast<void *>(ins[ks[i]]), outs[ks[i]], p.scratch};
+            sfh_tile_args &a = eargs[i];
+            a = l.args;
+            a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
+            a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
+            if (counts && counts[ks[i]] < p.cfg.batch_count) {
+                cut_to_count(l, a, counts[ks[i]], p.cfg.batch_count);
+            }
+            nitems += a.ntiles;
+            lds = std::max(lds, a.lds_bytes);
         }
         if (nitems > 0xffffffffull) {
             throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "queue execution has too many tiles");
@@ -2356,7 +2413,27 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         const size_t args_off = 16;
         const size_t items_off = args_off + ks.size() * sizeof(sfh_tile_args);
         const size_t bytes = items_off + nitems * sizeof(sfh_queue_item);
-        queue_slot &sl = qc.slot[qc.next++ % queue_ctx::k_slots];
+        // the next slot whose launch has completed; the host waits only when every slot is
+        // still in flight
+        int pick = -1;
+        for (int j = 0; j < queue_ctx::k_slots && pick < 0; ++j) {
+            const int c = (qc.next + j) % queue_ctx::k_slots;
+            if (!qc.slot[c].used) {
+                pick = c;
+            } else {
+                const hipError_t q = hipEventQuery(qc.slot[c].free);
+                if (q == hipSuccess) {
+                    pick = c;
+                } else if (q != hipErrorNotReady) {
+                    hip_check(q, "hipEventQuery(queue slot)");
+                }
+            }
+        }
+        if (pick < 0) {
+            pick = qc.next % queue_ctx::k_slots;
+        }
+        qc.next = pick + 1;
+        queue_slot &sl = qc.slot[pick];
         if (sl.used) {
             hip_check(hipEventSynchronize(sl.free), "hipEventSynchronize(queue slot)");
         }
@@ -2383,14 +2460,8 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         sfh_queue_item *items = reinterpret_cast<sfh_queue_item *>(h + items_off);
         uint64_t n = 0;
         for (size_t i = 0; i < ks.size(); ++i) {
-            const plan &p = *plans[ks[i]];
-            const launch &l = p.launches[0];
-            void *bufs[3] = {const_cast<void *>(ins[ks[i]]), outs[ks[i]], p.scratch};
-            sfh_tile_args a = l.args;
-            a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
-            a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
-            std::memcpy(&args[i], &a, sizeof(a));
-            for (uint64_t t = 0; t < a.ntiles; ++t) {
+            std::memcpy(&args[i], &eargs[i], sizeof(sfh_tile_args));
+            for (uint64_t t = 0; t < eargs[i].ntiles; ++t) {
                 items[n++] = sfh_queue_item{static_cast<uint32_t>(i), static_cast<uint32_t>(t)};
             }
         }
@@ -2603,6 +2674,35 @@ extern "C" int syclfft_hip_execute_queue(
         return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
     } catch (...) {
         sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+// Serving loop: class of `count` transforms of a plan for queue launches (-1: cannot
+// join one; equal classes share a launch), and a queue execution of checked buffers.
+extern "C" __attribute__((visibility("hidden"))) int sfh_plan_queue_class(const void *plan, uint64_t count)
+{
+    const sfh::plan &p = *static_cast<const sfh::plan *>(plan);
+    if (!sfh::queueable_for(p, count)) {
+        return -1;
+    }
+    return (p.dbl ? 1 << 16 : 0) + p.launches[0].variant;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int sfh_execute_queue_checked_buffers(
+    void *const *plans, const uint64_t *counts, const void *const *ins, void *const *outs, size_t n, void *stream,
+    void *done, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        sfh::execute_queue(reinterpret_cast<sfh::plan *const *>(plans), ins, outs, n, static_cast<hipStream_t>(stream),
+                           static_cast<hipEvent_t>(done), counts, false, false);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
         return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
     }
 }

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -41,6 +42,12 @@
 extern "C" int sfh_plan_execute_many_checked_buffers(
     void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *done, char *err,
     size_t cap);
+// fft_plan.cpp: work-queue class of `count` transforms of a plan (-1: none) and a queue
+// execution of checked buffers (entry k: counts[k] transforms of plans[k])
+extern "C" int sfh_plan_queue_class(const void *plan, uint64_t count);
+extern "C" int sfh_execute_queue_checked_buffers(
+    void *const *plans, const uint64_t *counts, const void *const *ins, void *const *outs, size_t n, void *stream,
+    void *done, char *err, size_t cap);
 
 namespace sfh_srv
 {
@@ -94,7 +101,7 @@ struct plan_entry
 struct launch_rec
 {
     std::vector<std::shared_ptr<request>> reqs;
-    plan_entry *pe{};
+    std::vector<plan_entry *> pes; // plans the launch runs (one per request in queue launches)
     hipEvent_t done{};
     uint64_t bytes{};
     size_t stream{};
@@ -131,6 +138,9 @@ public:
             throw srv_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "memory budget must be positive");
         }
         device_ = device;
+        if (const char *q = std::getenv("SYCLFFT_HIP_SERVE_QUEUE")) {
+            queue_launches_ = std::atoi(q) != 0;
+        }
         if (hipSetDevice(device_) != hipSuccess) {
             throw srv_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, "hipSetDevice failed");
         }
@@ -142,7 +152,12 @@ public:
             }
             streams_.push_back(s);
         }
+        // launches in flight before waiting requests stay on the host, where they can merge
+        // (SYCLFFT_HIP_SERVE_INFLIGHT overrides 4 per stream)
         max_inflight_ = 4 * nstreams;
+        if (const char *m = std::getenv("SYCLFFT_HIP_SERVE_INFLIGHT")) {
+            max_inflight_ = std::max<size_t>(1, std::strtoul(m, nullptr, 10));
+        }
         stream_load_.assign(nstreams, 0);
         dispatcher_ = std::thread([this] { dispatch_loop(); });
         completer_ = std::thread([this] { complete_loop(); });
@@ -466,16 +481,66 @@ private:
                 return true;
             }
         }
+        // Work queue: waiting requests of other shapes whose resident plans share the head
+        // plan's work-queue class (one plain tile launch of the same kernel variant) join the
+        // launch, which then runs every request's tiles from one persistent launch.
+        std::vector<plan_entry *> entry_plans; // per request, when the launch is a queue launch
+        if (pe && queue_launches_) {
+            int cls = -1;
+            bool all = true;
+            for (auto &q : group) {
+                const int c = sfh_plan_queue_class(pe->plan, q->r.batch_count);
+                all = all && c >= 0 && (cls < 0 || c == cls);
+                cls = c;
+            }
+            std::vector<std::shared_ptr<request>> extra;
+            std::vector<plan_entry *> extra_pe;
+            size_t looked = 0;
+            for (auto it = pending_.begin(); all && it != pending_.end() && looked < 1024 && extra.size() < 255;
+                 ++it, ++looked) {
+                const auto &q = *it;
+                if (q->key == head->key || footprint_ + bytes + q->bytes > budget_ || bytes + q->bytes > budget_ / 2) {
+                    continue; // same shape: in the group already, or beyond its plan's capacity
+                }
+                const auto f = by_key_.find(q->key);
+                if (f == by_key_.end()) {
+                    continue;
+                }
+                plan_entry *best = nullptr;
+                for (plan_entry *e : f->second) {
+                    if (e->capacity >= q->r.batch_count && (!best || e->inflight < best->inflight)) {
+                        best = e;
+                    }
+                }
+                if (best && sfh_plan_queue_class(best->plan, q->r.batch_count) == cls) {
+                    extra.push_back(q);
+                    extra_pe.push_back(best);
+                    bytes += q->bytes;
+                }
+            }
+            if (!extra.empty()) {
+                entry_plans.assign(group.size(), pe);
+                for (size_t k = 0; k < extra.size(); ++k) {
+                    group.push_back(extra[k]);
+                    entry_plans.push_back(extra_pe[k]);
+                }
+            }
+        }
         // reserve: take the group out of the queue, charge its footprint
         for (auto &q : group) {
             pending_.erase(std::find(pending_.begin(), pending_.end(), q));
         }
         footprint_ += bytes + plan_bytes;
         st_.peak_footprint_bytes = std::max(st_.peak_footprint_bytes, footprint_);
-        if (pe) {
-            ++pe->inflight;
-        }
         launch_rec rec;
+        if (!entry_plans.empty()) {
+            rec.pes = entry_plans;
+        } else if (pe) {
+            rec.pes.push_back(pe);
+        }
+        for (plan_entry *e : rec.pes) {
+            ++e->inflight;
+        }
         rec.reqs = group;
         rec.bytes = bytes;
         // the stream with the least work in flight (bytes of the launches queued on it; round
@@ -525,7 +590,32 @@ private:
                 msg = err;
             }
         }
-        if (status == SYCLFFT_HIP_STATUS_OK) {
+        if (status == SYCLFFT_HIP_STATUS_OK && !entry_plans.empty()) {
+            std::vector<void *> plans;
+            std::vector<uint64_t> counts;
+            std::vector<const void *> ins;
+            std::vector<void *> outs;
+            for (size_t k = 0; k < group.size(); ++k) {
+                plans.push_back(entry_plans[k]->plan);
+                counts.push_back(group[k]->r.batch_count);
+                ins.push_back(group[k]->r.in);
+                outs.push_back(group[k]->r.out);
+            }
+            char err[1024];
+            rec.done = take_event();
+            if (!rec.done) {
+                status = SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+                msg = "hipEventCreate failed";
+            } else {
+                status = sfh_execute_queue_checked_buffers(plans.data(), counts.data(), ins.data(), outs.data(),
+                                                           plans.size(), stream, rec.done, err, sizeof(err));
+                if (status != SYCLFFT_HIP_STATUS_OK) {
+                    msg = err;
+                    give_event(rec.done);
+                    rec.done = nullptr;
+                }
+            }
+        } else if (status == SYCLFFT_HIP_STATUS_OK) {
             std::vector<const void *> ins;
             std::vector<void *> outs;
             const uint64_t es = head->r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
@@ -561,14 +651,17 @@ private:
         } else if (!pe) {
             footprint_ -= plan_bytes; // plan creation failed
         }
-        if (pe) {
-            pe->last_use = t;
+        if (rec.pes.empty() && pe) {
+            rec.pes.push_back(pe); // the plan created for this launch (inflight counted at creation)
+        }
+        for (plan_entry *e : rec.pes) {
+            e->last_use = t;
         }
         if (status != SYCLFFT_HIP_STATUS_OK) {
             footprint_ -= bytes;
             stream_load_[si] -= bytes;
-            if (pe) {
-                --pe->inflight;
+            for (plan_entry *e : rec.pes) {
+                --e->inflight;
             }
             fail(group, status, msg);
             cv_dispatch_.notify_one();
@@ -578,7 +671,6 @@ private:
             q->t_launch = t;
             q->group = static_cast<uint32_t>(group.size());
         }
-        rec.pe = pe;
         inflight_.push_back(std::move(rec));
         ++st_.launches;
         cv_complete_.notify_one();
@@ -662,8 +754,10 @@ private:
                 }
                 footprint_ -= it->bytes;
                 stream_load_[it->stream] -= it->bytes;
-                --it->pe->inflight;
-                it->pe->last_use = t;
+                for (plan_entry *e : it->pes) {
+                    --e->inflight;
+                    e->last_use = t;
+                }
                 give_event(it->done);
                 inflight_.erase(it);
                 any = true;
@@ -681,6 +775,8 @@ private:
     }
 
     int device_{};
+    // merge requests of other shapes into work-queue launches (SYCLFFT_HIP_SERVE_QUEUE=0: off)
+    bool queue_launches_{true};
     int64_t t0_{now_ns()};
     uint64_t budget_, max_batch_;
     size_t max_inflight_{};

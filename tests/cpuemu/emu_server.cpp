@@ -202,6 +202,33 @@ int main()
         std::snprintf(d, sizeof(d), "relL2(alone, mix)=%.3e group=%u", ea, comp.group_size);
         check(wt == 0 && comp.group_size == 1 && ea <= tol_of(c), "alone vs mix " + std::to_string(k), d);
     }
+    // the mix again with every plan resident: waiting requests of other shapes whose plans
+    // share a kernel variant now merge into work-queue launches; results as in the first mix
+    {
+        for (auto &c : cases) {
+            reload(c);
+        }
+        std::vector<uint64_t> t2;
+        for (size_t k = 0; k < cases.size(); ++k) {
+            const syclfft_hip_request_v1 r = make_req(cases[k], 2000 + k);
+            uint64_t t = 0;
+            const int st = syclfft_hip_server_submit(srv, &r, &t, err, sizeof(err));
+            check(st == 0, "resubmit " + std::to_string(k), err);
+            t2.push_back(t);
+        }
+        uint32_t g2 = 0;
+        for (size_t k = 0; k < cases.size(); ++k) {
+            syclfft_hip_completion_v1 c{};
+            const int st = syclfft_hip_server_wait(srv, t2[k], &c, err, sizeof(err));
+            g2 = std::max(g2, c.group_size);
+            const std::vector<double> again = download(cases[k]);
+            const double e = rel_l2(again, cases[k].mix);
+            char d[128];
+            std::snprintf(d, sizeof(d), "relL2(second mix, first mix)=%.3e group=%u", e, c.group_size);
+            check(st == 0 && e <= tol_of(cases[k]), "second mix " + std::to_string(k), d);
+        }
+        check(g2 > 1, "second mix merged requests", "max_group=" + std::to_string(g2));
+    }
     // error paths
     {
         req_case big;
