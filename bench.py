@@ -355,6 +355,36 @@ def queue_bench(sh, torch, dev, nplans=256, reps=20, seed=20260007):
     return out
 
 
+def real_bench(sh, torch, dev, reps=10):
+    """Real 1-D plans: R2C and C2R, fp32 N=4096 batch 16384 and fp64 N=4096 batch 8192
+    (256 MiB of real values), time per execute and algorithmic GB/s (real side N values,
+    complex side N/2 + 1 values, one read and one write)."""
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    for name, n, b, dbl in (("f32_4096x16384", 4096, 16384, False), ("f64_4096x8192", 4096, 8192, True)):
+        rdt, cdt = (torch.float64, torch.complex128) if dbl else (torch.float32, torch.complex64)
+        x = torch.empty(n * b, dtype=rdt, device=dev).uniform_(-1, 1)
+        y = torch.empty(b * (n // 2 + 1), dtype=cdt, device=dev)
+        z = torch.empty_like(x)
+        bytes_ = n * b * x.element_size() + y.numel() * y.element_size()
+        for direction, src, dst, tag in ((sh.FORWARD, x, y, "r2c"), (sh.BACKWARD, y, z, "c2r")):
+            p = sh.Plan([n], b, direction, double=dbl, real=True, measure=True)
+            for _ in range(2):
+                p.execute(src, dst)
+            a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                p.execute(src, dst)
+            b_.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b_) / reps
+            out[f"{tag}_{name}"] = {"ms": round(ms, 4), "gbs": round(bytes_ / (ms * 1e-3) / 1e9, 1),
+                                    "launches": p.launch_count()}
+            p.close()
+        del x, y, z
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -549,6 +579,7 @@ def main():
 
     # ---------------- multi-D configuration
     multid = None if args.no_sweep else multid_bench(sh, torch, dev)
+    real = None if args.no_sweep else real_bench(sh, torch, dev)
 
     # ---------------- precision configuration: plain FP32 VALU vs split-precision MFMA
     precision = None if args.no_precision else precision_bench(sh, torch, dev, lib)
@@ -607,6 +638,7 @@ def main():
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
         "multid": multid,
+        "real": real,
         "precision": precision,
         "work_queue": work_queue,
         "serving": serving,

@@ -46,5 +46,10 @@ extern "C" int sfh_resident_groups(const sfh_kernel_variant *variant, int mode_i
 extern "C" int sfh_launch_queue(
     const sfh_kernel_variant *variant, const sfh_queue_args *args, uint32_t grid, uint32_t lds_bytes,
     hipStream_t stream);
+// real transforms: R2C untangle (c2r = 0: M complex -> M + 1 per transform) or C2R tangle
+// (c2r = 1: M + 1 -> M), tw = W_N^k, k = 0..M, N = 2M
+extern "C" int sfh_launch_real_pass(
+    int c2r, const void *in, void *out, const void *tw, uint64_t M, uint64_t batch, int is_double, double scale,
+    hipStream_t stream);
 extern "C" int sfh_launch_copy_scale(
     const void *in, void *out, uint64_t count, int is_double, double scale, hipStream_t stream);

@@ -289,6 +289,14 @@ struct plan
     int copy_only{}; // all lengths 1: a scaled copy
     int cap_limit{};  // largest tile (complex elements) the planner may use; 0 = no limit
     int split_launches{}; // launches whose radix-16 passes run on MFMA (split precision)
+    // real 1-D plans (syclfft_hip_real_plan_create): 1 = R2C, 2 = C2R of even length N = 2 real_m,
+    // run as the complex plan `half` of length real_m over the real buffer viewed as complex,
+    // plus one untangle / tangle pass with the table W_N^k (k = 0..real_m) into / out of scratch
+    int real{};
+    struct plan *half{};
+    uint64_t real_m{};
+    void *real_tw{};
+    void *real_scratch{};
     // unit-table ring for executions over pointer arrays (allocated on first use): per slot
     // a device table block, its pinned host staging copy, and the event after which the
     // slot's last execution no longer reads it
@@ -1466,6 +1474,15 @@ static void release(plan *p)
     if (p->last) {
         (void)hipEventDestroy(p->last);
     }
+    if (p->half) {
+        release(p->half);
+    }
+    if (p->real_tw) {
+        (void)hipFree(p->real_tw);
+    }
+    if (p->real_scratch) {
+        (void)hipFree(p->real_scratch);
+    }
     if (p->scratch) {
         (void)hipFree(p->scratch);
     }
@@ -1969,10 +1986,17 @@ static void check_buffers(const plan &p, const void *in, const void *out)
     }
 }
 
+static void execute_real(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done);
+
 static void execute_plan(
     plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
     hipEvent_t done)
 {
+    if (p.real) {
+        execute_real(p, in, out, stream, deps, ndeps, done);
+        return;
+    }
     check_buffers(p, in, out);
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
@@ -2498,9 +2522,128 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
     }
 }
 
+// Real 1-D transform: R2C = complex plan of length M over the real input viewed as M complex
+// values, then the untangle pass into the M + 1 outputs; C2R = the tangle pass into scratch,
+// then the complex backward plan writing the real output viewed as M complex values.
+static void execute_real(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done)
+{
+    const uint64_t B = p.cfg.batch_count, M = p.real_m;
+    const uint64_t real_bytes = B * M * p.esize;       // B N real values
+    const uint64_t half_bytes = B * (M + 1) * p.esize; // B (M + 1) complex values
+    const uint64_t in_bytes = p.real == 1 ? real_bytes : half_bytes;
+    const uint64_t out_bytes = p.real == 1 ? half_bytes : real_bytes;
+    validate_pointer(p, in, in_bytes, "input");
+    validate_pointer(p, out, out_bytes, "output");
+    const auto a = reinterpret_cast<uintptr_t>(in), b = reinterpret_cast<uintptr_t>(out);
+    if (a == b) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place plan requires distinct allocations");
+    }
+    if (a < b + out_bytes && b < a + in_bytes) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
+    }
+    std::lock_guard<std::mutex> lock(p.mu);
+    device_guard guard(p.device);
+    const bool capturing = stream_capturing(stream);
+    for (size_t i = 0; i < ndeps; ++i) {
+        if (deps[i]) {
+            hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
+        }
+    }
+    if (!capturing) {
+        wait_last(p, stream);
+    }
+    if (p.real == 1) {
+        execute_plan(*p.half, in, p.real_scratch, stream, nullptr, 0, nullptr);
+        hip_check(static_cast<hipError_t>(sfh_launch_real_pass(0, p.real_scratch, out, p.real_tw, M, B, p.dbl ? 1 : 0,
+                                                               p.scale, stream)),
+                  "untangle launch");
+    } else {
+        hip_check(static_cast<hipError_t>(sfh_launch_real_pass(1, in, p.real_scratch, p.real_tw, M, B, p.dbl ? 1 : 0,
+                                                               p.scale, stream)),
+                  "tangle launch");
+        execute_plan(*p.half, p.real_scratch, out, stream, nullptr, 0, nullptr);
+    }
+    if (!capturing) {
+        hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+        set_last(p, stream);
+    }
+    if (done) {
+        hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+    }
+}
+
+static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
+{
+    validate_config(cfg);
+    if (cfg->rank != 1 || cfg->lengths[0] < 2 || cfg->lengths[0] % 2 != 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "real plans are 1-D with an even length >= 2");
+    }
+    if (cfg->in_place) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                        "real plans are out of place (N real values and N/2 + 1 complex values differ in size)");
+    }
+    if (cfg->precision != SYCLFFT_HIP_PRECISION_VALU) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_UNSUPPORTED_PRECISION, "real plans use the VALU precision mode");
+    }
+    syclfft_hip_plan_config_v1 h = *cfg;
+    h.lengths[0] = cfg->lengths[0] / 2;
+    h.normalization = SYCLFFT_HIP_NORM_NONE;
+    std::unique_ptr<plan, void (*)(plan *)> hp(create_plan(&h), release);
+    std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
+    p->cfg = *cfg;
+    p->device = hp->device;
+    p->dbl = hp->dbl;
+    p->esize = hp->esize;
+    p->variants = hp->variants;
+    p->nvariants = hp->nvariants;
+    p->tune = hp->tune;
+    p->transform = cfg->lengths[0];
+    p->total = p->transform * cfg->batch_count;
+    p->scale = norm_scale(cfg->normalization, cfg->direction, p->transform);
+    p->real = cfg->direction == -1 ? 1 : 2;
+    p->real_m = h.lengths[0];
+    const uint64_t M = p->real_m, N = cfg->lengths[0];
+    device_guard guard(p->device);
+    std::vector<unsigned char> host((M + 1) * p->esize);
+    for (uint64_t k = 0; k <= M; ++k) {
+        const cxl w = root_ld(k, N);
+        if (p->dbl) {
+            double *d = reinterpret_cast<double *>(host.data()) + 2 * k;
+            d[0] = static_cast<double>(w.re);
+            d[1] = static_cast<double>(w.im);
+        } else {
+            float *f = reinterpret_cast<float *>(host.data()) + 2 * k;
+            f[0] = static_cast<float>(w.re);
+            f[1] = static_cast<float>(w.im);
+        }
+    }
+    hip_check(hipMalloc(&p->real_tw, host.size()), "hipMalloc(real twiddles)");
+    hip_check(hipMemcpy(p->real_tw, host.data(), host.size(), hipMemcpyHostToDevice), "hipMemcpy(real twiddles)");
+    hip_check(hipMalloc(&p->real_scratch, cfg->batch_count * M * p->esize), "hipMalloc(real scratch)");
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
+    p->half = hp.release();
+    trace(std::string(p->real == 1 ? "real r2c" : "real c2r") + " n=" + std::to_string(N) + " over complex n=" +
+          std::to_string(M));
+    return p.release();
+}
+
 static std::string describe_plan(const plan &p)
 {
     std::string s;
+    if (p.real) {
+        const std::string pass = p.real == 1 ? "r2c-untangle" : "c2r-tangle";
+        if (p.real == 2) {
+            s += pass + " m=" + std::to_string(p.real_m) + "\n";
+        }
+        const std::string h = describe_plan(*p.half);
+        s += h.substr(0, h.rfind("tuning"));
+        if (p.real == 1) {
+            s += pass + " m=" + std::to_string(p.real_m) + "\n";
+        }
+        s += "tuning " + p.half->tune.str() + "\n";
+        return s;
+    }
     if (p.copy_only) {
         s += "copy_scale\n";
     }
@@ -2582,6 +2725,32 @@ extern "C" int syclfft_hip_plan_create_ex(
     return SYCLFFT_HIP_STATUS_PLANNING_FAILED;
 }
 
+extern "C" int syclfft_hip_real_plan_create(
+    const syclfft_hip_plan_config_v1 *config, void **plan, char *err, size_t cap)
+{
+    if (plan) {
+        *plan = nullptr;
+    }
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plan output pointer is null");
+        }
+        *plan = sfh::create_real_plan(config);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::bad_alloc &) {
+        sfh::set_err(err, cap, "Unable to allocate plan memory");
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+    }
+    return SYCLFFT_HIP_STATUS_PLANNING_FAILED;
+}
+
 extern "C" void syclfft_hip_plan_destroy(void *plan)
 {
     sfh::release(static_cast<sfh::plan *>(plan));
@@ -2593,6 +2762,9 @@ extern "C" uint64_t syclfft_hip_plan_scratch_bytes(const void *plan)
         return 0;
     }
     const auto *p = static_cast<const sfh::plan *>(plan);
+    if (p->real) {
+        return p->cfg.batch_count * p->real_m * p->esize + syclfft_hip_plan_scratch_bytes(p->half);
+    }
     return p->scratch_elems * p->esize;
 }
 
@@ -2745,6 +2917,9 @@ extern "C" uint32_t syclfft_hip_plan_launch_count(const void *plan)
         return 0;
     }
     const auto *p = static_cast<const sfh::plan *>(plan);
+    if (p->real) {
+        return 1u + syclfft_hip_plan_launch_count(p->half);
+    }
     return static_cast<uint32_t>(p->launches.size()) + (p->copy_only ? 1u : 0u);
 }
 

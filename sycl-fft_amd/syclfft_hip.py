@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_supports",
     "syclfft_hip_plan_create",
     "syclfft_hip_plan_create_ex",
+    "syclfft_hip_real_plan_create",
     "syclfft_hip_plan_destroy",
     "syclfft_hip_plan_scratch_bytes",
     "syclfft_hip_plan_execute",
@@ -129,6 +130,9 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.syclfft_hip_plan_create_ex.argtypes = [
         ctypes.POINTER(PlanConfig), ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
     lib.syclfft_hip_plan_create_ex.restype = ctypes.c_int
+    lib.syclfft_hip_real_plan_create.argtypes = [ctypes.POINTER(PlanConfig), ctypes.POINTER(ctypes.c_void_p),
+                                                 ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_real_plan_create.restype = ctypes.c_int
     lib.syclfft_hip_plan_destroy.argtypes = [ctypes.c_void_p]
     lib.syclfft_hip_plan_destroy.restype = None
     lib.syclfft_hip_plan_scratch_bytes.argtypes = [ctypes.c_void_p]
@@ -403,8 +407,12 @@ class Plan:
 
     def __init__(self, lengths: Sequence[int], batch: int = 1, direction: int = FORWARD,
                  double: bool = False, in_place: bool = False, normalization: str = "none",
-                 precision: str = "valu", measure: bool = False, device: int = -1):
+                 precision: str = "valu", measure: bool = False, device: int = -1, real: bool = False):
+        """real=True: a real 1-D plan (syclfft_hip_real_plan_create): FORWARD = R2C (N real ->
+        N/2 + 1 complex per transform), BACKWARD = C2R; execute on float/double tensors for
+        the real side and complex tensors for the spectrum side."""
         self._lib = load()
+        self.real = bool(real)
         self.lengths = tuple(int(n) for n in lengths)
         self.batch = int(batch)
         self.direction = int(direction)
@@ -417,7 +425,8 @@ class Plan:
         err = ctypes.create_string_buffer(1024)
         h = ctypes.c_void_p()
         self._h = None
-        st = self._lib.syclfft_hip_plan_create_ex(ctypes.byref(cfg), ctypes.byref(h), err, 1024)
+        create = self._lib.syclfft_hip_real_plan_create if real else self._lib.syclfft_hip_plan_create_ex
+        st = create(ctypes.byref(cfg), ctypes.byref(h), err, 1024)
         if st != 0:
             _raise(st, err)
         self._h = h

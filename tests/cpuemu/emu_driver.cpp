@@ -22,8 +22,135 @@
 extern "C" void oracle_fftn_ld(int rank, const uint64_t *lengths, uint64_t batch, const double *in,
                                double *out, int sign, int norm);
 
+// Real plans ('R' float, 'Q' double), N even: direction -1 = R2C (N real -> N/2 + 1 complex),
+// +1 = C2R (a Hermitian half spectrum -> N real); checked against the complex FP64 oracle.
+static bool run_real(const std::string &spec)
+{
+    char prec = 0;
+    unsigned long long n = 0, batch = 1;
+    int dir = -1, norm = 0, inplace = 0;
+    if (std::sscanf(spec.c_str(), "%c:%llu:%llu:%d:%d:%d", &prec, &n, &batch, &dir, &norm, &inplace) != 6) {
+        std::printf("FAIL %s bad case spec\n", spec.c_str());
+        return false;
+    }
+    const bool dbl = prec == 'Q';
+    const uint64_t M = n / 2;
+    syclfft_hip_plan_config_v1 cfg{};
+    cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
+    cfg.scalar = dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
+    cfg.direction = dir;
+    cfg.rank = 1;
+    cfg.lengths[0] = n;
+    cfg.batch_count = batch;
+    cfg.normalization = static_cast<uint32_t>(norm);
+    cfg.device = -1;
+    char err[1024];
+    void *plan = nullptr;
+    if (syclfft_hip_real_plan_create(&cfg, &plan, err, sizeof(err)) != 0) {
+        std::printf("FAIL %s create: %s\n", spec.c_str(), err);
+        return false;
+    }
+    std::mt19937_64 rng(20261100 + n + batch);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    // a real signal r (batch x n) and its full spectrum R (FP64 oracle, no normalization)
+    std::vector<double> rc(2 * n * batch, 0.0), spec_full(2 * n * batch);
+    for (uint64_t i = 0; i < n * batch; ++i) {
+        const double v = u(rng);
+        rc[2 * i] = dbl ? v : static_cast<double>(static_cast<float>(v));
+    }
+    const uint64_t lens[1] = {n};
+    oracle_fftn_ld(1, lens, batch, rc.data(), spec_full.data(), -1, 0);
+    const size_t rs = dbl ? 8 : 4; // bytes per real value
+    const uint64_t in_vals = dir == -1 ? n * batch : 2 * (M + 1) * batch;
+    const uint64_t out_vals = dir == -1 ? 2 * (M + 1) * batch : n * batch;
+    void *din = nullptr, *dout = nullptr;
+    hipMalloc(&din, in_vals * rs);
+    hipMalloc(&dout, out_vals * rs);
+    std::vector<double> in(in_vals), expect(out_vals);
+    if (dir == -1) {
+        for (uint64_t i = 0; i < n * batch; ++i) {
+            in[i] = rc[2 * i];
+        }
+        std::vector<double> ref(2 * n * batch);
+        oracle_fftn_ld(1, lens, batch, rc.data(), ref.data(), -1, norm);
+        for (uint64_t b = 0; b < batch; ++b) {
+            for (uint64_t k = 0; k <= M; ++k) {
+                expect[2 * (b * (M + 1) + k)] = ref[2 * (b * n + k)];
+                expect[2 * (b * (M + 1) + k) + 1] = ref[2 * (b * n + k) + 1];
+            }
+        }
+    } else {
+        // the input: bins 0..M of the spectrum (rounded to the plan's type); the expected
+        // output: the oracle's inverse of the full Hermitian spectrum those bins define
+        std::vector<double> full(2 * n * batch), ref(2 * n * batch);
+        for (uint64_t b = 0; b < batch; ++b) {
+            for (uint64_t k = 0; k <= M; ++k) {
+                double re = spec_full[2 * (b * n + k)], im = spec_full[2 * (b * n + k) + 1];
+                if (!dbl) {
+                    re = static_cast<float>(re);
+                    im = static_cast<float>(im);
+                }
+                if (k == 0 || k == M) {
+                    im = 0.0; // a Hermitian spectrum is real at DC and Nyquist
+                }
+                in[2 * (b * (M + 1) + k)] = re;
+                in[2 * (b * (M + 1) + k) + 1] = im;
+                full[2 * (b * n + k)] = re;
+                full[2 * (b * n + k) + 1] = im;
+                if (k && k < M) {
+                    full[2 * (b * n + n - k)] = re;
+                    full[2 * (b * n + n - k) + 1] = -im;
+                }
+            }
+        }
+        oracle_fftn_ld(1, lens, batch, full.data(), ref.data(), 1, norm);
+        for (uint64_t i = 0; i < n * batch; ++i) {
+            expect[i] = ref[2 * i];
+        }
+    }
+    if (dbl) {
+        std::memcpy(din, in.data(), in_vals * 8);
+    } else {
+        float *f = static_cast<float *>(din);
+        for (uint64_t i = 0; i < in_vals; ++i) {
+            f[i] = static_cast<float>(in[i]);
+        }
+    }
+    if (syclfft_hip_plan_execute(plan, din, dout, nullptr, nullptr, 0, nullptr, err, sizeof(err)) != 0) {
+        std::printf("FAIL %s execute: %s\n", spec.c_str(), err);
+        return false;
+    }
+    double num = 0, den = 0;
+    for (uint64_t i = 0; i < out_vals; ++i) {
+        const double g = dbl ? static_cast<double *>(dout)[i] : static_cast<float *>(dout)[i];
+        num += (g - expect[i]) * (g - expect[i]);
+        den += expect[i] * expect[i];
+    }
+    const double rel = std::sqrt(num / (den > 0 ? den : 1));
+    const double u_ = dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(n))) * u_;
+    char desc[2048];
+    syclfft_hip_plan_describe(plan, desc, sizeof(desc));
+    for (char *c = desc; *c; ++c) {
+        if (*c == '\n') {
+            *c = '|';
+        }
+    }
+    const bool ok = rel <= tol && std::isfinite(rel);
+    std::printf("%s %s relL2=%.3e tol=%.1e launches=%u [%s]\n", ok ? "ok" : "FAIL", spec.c_str(), rel, tol,
+                syclfft_hip_plan_launch_count(plan), desc);
+    std::fflush(stdout);
+    syclfft_hip_plan_destroy(plan);
+    hipFree(din);
+    hipFree(dout);
+    return ok;
+}
+
 static bool run_case(const std::string &spec)
 {
+    if (!spec.empty() && (spec[0] == 'R' || spec[0] == 'Q')) {
+        return run_real(spec);
+    }
     char prec = 0;
     char dims[128] = {0};
     unsigned long long batch = 1;

@@ -149,3 +149,11 @@ def test_emulated_work_queue_execution():
           "f:10007:2:-1:0:0", "d:64:7:-1:0:1", "f:512:4:1:0:0", "f:1000:3:-1:0:0", "f:32x32:2:1:2:0",
           "f:65536:1:-1:0:0"], {"EMU_QUEUE": "1"})
 
+
+def test_emulated_real_plans():
+    """R2C (N real -> N/2 + 1 complex) and C2R plans ('R' float, 'Q' double) against the
+    complex FP64 oracle: the half-length complex plan over the real data viewed as complex,
+    single tile, Bluestein (N/2 prime) and four-step half plans, every normalization."""
+    _run(["R:2:3:-1:0:0", "R:16:4:-1:1:0", "R:4096:2:-1:0:0", "Q:2520:2:-1:3:0", "R:30:3:1:0:0",
+          "R:4096:2:1:2:0", "Q:1000:2:1:3:0", "R:20014:1:-1:0:0", "Q:2:2:1:0:0", "R:65536:1:1:1:0"])
+
