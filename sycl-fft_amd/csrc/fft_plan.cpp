@@ -2341,8 +2341,8 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
         if (counts && !queueable_for(p, counts[k])) {
             throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "entry cannot join a queue launch");
         }
-        if (!validate) {
-            continue;
+        if (!validate || (!counts && !queueable(p))) {
+            continue; // entries that execute directly are checked by their execute path
         }
         check(p, ins[k], "input");
         check(p, outs[k], "output");

@@ -72,6 +72,28 @@ MIXED = [
 ]
 
 
+def test_queue_with_real_plan_entry():
+    """A real (R2C) plan in a queue call executes directly with its own buffer sizes (N real
+    values in, N/2 + 1 complex values out) beside queued complex plans."""
+    n, batch = 512, 3
+    g = np.random.default_rng(20261500)
+    xr = g.uniform(-1, 1, n * batch).astype(np.float32)
+    rp = sh.Plan([n], batch, sh.FORWARD, real=True)
+    c = make_case([256], 4, -1, "none", False, False, seed=20261501)
+    xd = torch.from_numpy(xr).to("cuda")
+    yd = torch.empty(batch * (n // 2 + 1), dtype=torch.complex64, device="cuda")
+    cx = torch.from_numpy(c["x"].copy()).to("cuda")
+    cy = torch.empty_like(cx)
+    say("case: work queue with a real plan entry")
+    sh.execute_queue_ptr([rp, c["plan"]], [xd.data_ptr(), cx.data_ptr()], [yd.data_ptr(), cy.data_ptr()])
+    torch.cuda.synchronize()
+    ref = oracle_fftn(xr.astype(np.complex128), [n], batch, -1, "none").reshape(batch, n)[:, : n // 2 + 1]
+    assert rel_l2(yd.cpu().numpy(), ref) <= 4 * np.log2(n) * U[False]
+    assert np.array_equal(cy.cpu().numpy(), run_alone(c))
+    rp.close()
+    c["plan"].close()
+
+
 def test_queue_mixed_plans_bit_identical_to_alone_and_oracle():
     cases = [make_case(*m, seed=20260800 + k) for k, m in enumerate(MIXED)]
     say("case: work queue over " + ", ".join(f"{c['lengths']}x{c['batch']}{'d' if c['double'] else 'f'}"
