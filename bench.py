@@ -104,6 +104,120 @@ def cpu_baseline(lib, sample_scale=16, target_s=12.0):
     return total_flops / elapsed / 1e9, reps, sample
 
 
+class _HostCfg(ctypes.Structure):
+    # include/syclfft/detail/provider_abi.h (bit-compatible with the reference's host ABI v1)
+    _fields_ = [("abi_version", ctypes.c_uint32), ("scalar", ctypes.c_uint32), ("direction", ctypes.c_int32),
+                ("rank", ctypes.c_uint32), ("lengths", ctypes.c_uint64 * 3), ("batch_count", ctypes.c_uint64),
+                ("in_place", ctypes.c_uint32), ("measure", ctypes.c_uint32)]
+
+
+class _HostProvider(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_uint32), ("name", ctypes.c_char_p),
+                ("create", ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.POINTER(_HostCfg), ctypes.c_char_p, ctypes.c_size_t)),
+                ("destroy", ctypes.CFUNCTYPE(None, ctypes.c_void_p)),
+                ("execute", ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_char_p, ctypes.c_size_t))]
+
+
+def host_fftw_baseline(sample_scale=16, target_s=8.0):
+    """The reference's host path (host::plan over its FFTW module, src/host/host.cpp:37-137,
+    src/providers/fftw/fftw_provider.cpp:133-355): this repository's host provider module
+    over the FFTW3 API (libfftw3 if installed, else MKL's FFTW3 interface), single-threaded
+    as the reference configures FFTW, FFTW_ESTIMATE plans, on the same sample as
+    cpu_baseline; forward + backward without normalization.  Returns a dict or None."""
+    path = os.path.join(ROOT, "sycl-fft_amd", "lib", "syclfft", "providers", "syclfft_provider_fftw.so")
+    try:
+        mod = ctypes.CDLL(path)
+    except OSError:
+        return None
+    mod.syclfft_get_host_provider_v1.restype = ctypes.POINTER(_HostProvider)
+    mod.syclfft_fftw_provider_backend.restype = ctypes.c_char_p
+    prov = mod.syclfft_get_host_provider_v1().contents
+    err = ctypes.create_string_buffer(1024)
+    # the reference runs FFTW single-threaded; MKL's FFTW3 interface (if that backs the module;
+    # asking for the backend loads it) is set to one thread through its own call, before any
+    # plan exists, for this measurement's library only
+    backend = mod.syclfft_fftw_provider_backend().decode()
+    if backend.startswith("mkl"):
+        try:
+            mkl = ctypes.CDLL(backend[backend.index("(") + 1: backend.rindex(")")])
+            mkl.MKL_Set_Num_Threads(1)
+        except (OSError, AttributeError, ValueError):
+            pass
+    g = np.random.default_rng(7)
+    cases = []
+    for name, n, b, dbl in WORKLOAD:
+        bs = max(1, b // sample_scale)
+        x = (g.uniform(-1, 1, n * bs) + 1j * g.uniform(-1, 1, n * bs)).astype(np.complex128 if dbl else np.complex64)
+        plans = []
+        for direction in (-1, 1):
+            cfg = _HostCfg(1, 2 if dbl else 1, direction, 1, (ctypes.c_uint64 * 3)(n, 1, 1), bs, 0, 0)
+            h = prov.create(ctypes.byref(cfg), err, 1024)
+            if not h:
+                for q in plans:
+                    prov.destroy(q)
+                return None
+            plans.append(h)
+        cases.append((n, bs, x, np.empty_like(x), plans))
+    elapsed, fl, reps = 0.0, 0.0, 0
+    while elapsed < target_s and reps < 50:
+        for n, bs, x, y, (pf, pb) in cases:
+            t0 = time.perf_counter()
+            prov.execute(pf, x.ctypes.data, y.ctypes.data, err, 1024)
+            prov.execute(pb, y.ctypes.data, x.ctypes.data, err, 1024)
+            elapsed += time.perf_counter() - t0
+            fl += 2 * flops(n, bs)
+            x *= 1.0 / n  # keep the values bounded across repetitions (outside the timed calls)
+        reps += 1
+    for *_, plans in cases:
+        for q in plans:
+            prov.destroy(q)
+    sample = ", ".join(f"{'f64' if c[2].dtype == np.complex128 else 'f32'} N={c[0]} batch={c[1]}" for c in cases)
+    return {"value": round(fl / elapsed / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "library",
+            "library": backend,
+            "sample": f"host::plan path over the FFTW3 API, single-threaded, estimate planning, fwd+bwd, "
+                      f"{sample} (1/{sample_scale} of the workload's lines), {reps} reps"}
+
+
+def cache_bench(sh, torch, dev, reps=10):
+    """Working sets under 256 MiB, warm (back-to-back executions, data resident in the
+    caches) and cold (a 1 GiB write between executions evicts L2 and the MALL): the plain
+    fp32 N=4096 batch 256 forward and the Bluestein f32 N=10007 batch 128 forward."""
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    scrub = torch.empty(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+    for name, n, b, norm in (("c1_f32_4096x256", 4096, 256, "none"), ("c2_f32_10007x128", 10007, 128, "orthogonal")):
+        x = torch.view_as_complex((torch.rand((n * b, 2), device=dev) * 2 - 1).contiguous())
+        y = torch.empty_like(x)
+        p = sh.Plan([n], b, sh.FORWARD, normalization=norm, measure=True)
+        for _ in range(3):
+            p.execute(x, y)
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps * 2):
+            p.execute(x, y)
+        b_.record(stream)
+        torch.cuda.synchronize()
+        warm = a.elapsed_time(b_) / (reps * 2)
+        cold = []
+        for k in range(reps):
+            scrub.fill_(float(k))
+            a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            p.execute(x, y)
+            b_.record(stream)
+            torch.cuda.synchronize()
+            cold.append(a.elapsed_time(b_))
+        byts = 2 * n * b * 8
+        out[name] = {"warm_ms": round(warm, 4), "cold_ms": round(float(np.median(cold)), 4),
+                     "warm_gbs": round(byts / (warm * 1e-3) / 1e9, 1),
+                     "cold_gbs": round(byts / (float(np.median(cold)) * 1e-3) / 1e9, 1)}
+        p.close()
+        del x, y
+    del scrub
+    return out
+
+
 def smooth7(n):
     for f in (2, 3, 5, 7):
         while n % f == 0:
@@ -597,8 +711,13 @@ def main():
         serving = serving_bench(sh, torch, dev, nreq=args.serve_requests)
         log(f"serving: {serving}")
 
+    # ---------------- warm (cache-resident) vs cold timings of the small working sets (after
+    # the serving loop: its 1 GiB scrub buffer stays in the allocator's cache)
+    caches = None if args.no_sweep else cache_bench(sh, torch, dev)
+
     # ---------------- CPU baseline (reference algorithm, restated; host cores)
     cpu = None
+    host_fftw = None
     if not args.no_cpu:
         gf, reps, sample = cpu_baseline(lib)
         cpu = {"value": round(gf, 3), "unit": "GFLOP/s", "cores": int(lib.oracle_max_threads()),
@@ -606,6 +725,7 @@ def main():
                "sample": f"reference portable-provider algorithm restated in C/OpenMP "
                          f"(oracle/oracle.c ref_portable_exec_*), fwd+bwd orthogonal, {sample} "
                          f"(1/16 of the workload's lines), {reps} reps"}
+        host_fftw = host_fftw_baseline()
 
     line = {
         "metric": "C2C GFLOP/s (5N*log2N), mixed-radix + Bluestein round trip",
@@ -635,10 +755,12 @@ def main():
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                      "scaled by the copy-kernel factor)"},
         "cpu_baseline": cpu,
+        "cpu_baseline_host_path": host_fftw,
         "accuracy": acc,
         "fp32_hbm_sweep": sweep,
         "multid": multid,
         "real": real,
+        "warm_cold": caches,
         "precision": precision,
         "work_queue": work_queue,
         "serving": serving,
