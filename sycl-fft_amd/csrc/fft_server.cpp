@@ -141,6 +141,12 @@ public:
         if (const char *q = std::getenv("SYCLFFT_HIP_SERVE_QUEUE")) {
             queue_launches_ = std::atoi(q) != 0;
         }
+        if (const char *q = std::getenv("SYCLFFT_HIP_SERVE_SJF")) {
+            sjf_ = std::atoi(q) != 0;
+        }
+        if (const char *q = std::getenv("SYCLFFT_HIP_SERVE_SJF_AGE_US")) {
+            sjf_age_ns_ = 1000 * static_cast<int64_t>(std::strtoll(q, nullptr, 10));
+        }
         if (hipSetDevice(device_) != hipSuccess) {
             throw srv_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, "hipSetDevice failed");
         }
@@ -390,13 +396,27 @@ private:
     // for a completion (budget or slots).
     bool launch_next(std::unique_lock<std::mutex> &lk)
     {
+        // The oldest request leads the next launch; with size-first dispatch
+        // (SYCLFFT_HIP_SERVE_SJF=1) the smallest of the first 64 waiting requests does, until
+        // the oldest has waited sjf_age_ns_ (no request waits behind smaller ones for longer)
         std::shared_ptr<request> head = pending_.front();
+        if (sjf_ && now_ns() - head->t_submit < sjf_age_ns_) {
+            size_t looked = 0;
+            for (auto it = pending_.begin(); it != pending_.end() && looked < 64; ++it, ++looked) {
+                if ((*it)->bytes < head->bytes) {
+                    head = *it;
+                }
+            }
+        }
         const uint64_t cap = std::max<uint64_t>(max_batch_, head->r.batch_count);
         std::vector<std::shared_ptr<request>> group{head};
         uint64_t units = head->r.batch_count, bytes = head->bytes;
         size_t scanned = 0;
-        for (auto it = std::next(pending_.begin()); it != pending_.end() && scanned < 1024; ++it, ++scanned) {
+        for (auto it = pending_.begin(); it != pending_.end() && scanned < 1024; ++it, ++scanned) {
             const auto &q = *it;
+            if (q == head) {
+                continue;
+            }
             if (q->key == head->key && units + q->r.batch_count <= cap && bytes + q->bytes <= budget_ / 2) {
                 group.push_back(q);
                 units += q->r.batch_count;
@@ -476,7 +496,7 @@ private:
                 }
             }
             if (group.size() > 1 || !make_room(bytes + plan_bytes)) {
-                pending_.pop_front();
+                pending_.erase(std::find(pending_.begin(), pending_.end(), head));
                 fail(group, SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "request and its plan exceed the memory budget");
                 return true;
             }
@@ -777,6 +797,9 @@ private:
     int device_{};
     // merge requests of other shapes into work-queue launches (SYCLFFT_HIP_SERVE_QUEUE=0: off)
     bool queue_launches_{true};
+    // size-first dispatch with an age bound (SYCLFFT_HIP_SERVE_SJF, _SJF_AGE_US)
+    bool sjf_{false};
+    int64_t sjf_age_ns_{500000};
     int64_t t0_{now_ns()};
     uint64_t budget_, max_batch_;
     size_t max_inflight_{};
