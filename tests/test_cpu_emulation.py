@@ -112,16 +112,19 @@ def test_emulated_pointer_array_execution(mode):
 EMU_SERVER = os.path.join(os.path.dirname(EMU), "emu_server")
 
 
-def test_emulated_serving_loop():
+@pytest.mark.parametrize("env", [{}, {"SYCLFFT_HIP_SERVE_SJF": "1", "SYCLFFT_HIP_SERVE_SJF_AGE_US": "200"},
+                                 {"SYCLFFT_HIP_SERVE_QUEUE": "0"}])
+def test_emulated_serving_loop(env):
     """The serving loop on the emulated device: a mix of 36 requests (6 shapes, both
     types, directions, normalizations, in-place and out-of-place) submitted at once is
     merged into shared launches, each result matches the FP64 oracle and the same request
     run alone; budget, ticket and host-buffer errors are rejected."""
     if not os.path.exists(EMU_SERVER):
         pytest.fail(f"{EMU_SERVER} is not built (run make -C tests/cpuemu)")
-    env = dict(os.environ)
-    env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=0"
-    r = subprocess.run([EMU_SERVER], capture_output=True, text=True, timeout=900, env=env)
+    run_env = dict(os.environ)
+    run_env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=0"
+    run_env.update(env)  # default dispatch, size-first dispatch, no work-queue merging
+    r = subprocess.run([EMU_SERVER], capture_output=True, text=True, timeout=900, env=run_env)
     lines = r.stdout.splitlines()
     bad = [l for l in lines if not l.startswith("ok ")]
     assert r.returncode == 0 and not bad and len(lines) > 100, (r.stdout[-4000:], r.stderr[-4000:])
