@@ -499,6 +499,44 @@ def real_bench(sh, torch, dev, reps=10):
     return out
 
 
+def layout_bench(sh, torch, dev, reps=10):
+    """Layout plans against the contiguous plan, fp32 N=4096 batch 8192 (256 MiB each way):
+    rows padded to a distance of N + 64, interleaved batches (stride = batch, distance 1),
+    both fused into the one tile launch, and planar (split-complex) input and output, which
+    gather into scratch, run the contiguous plan in place and scatter.  Time per execute and
+    algorithmic GB/s (one read and one write of the 2 N batch values)."""
+    out = {}
+    stream = torch.cuda.current_stream(dev)
+    n, b = 4096, 8192
+    bytes_ = 2 * n * b * 8
+    cases = (("contiguous", None), ("padded_rows", sh.Layout(1, n + 64, 1, n + 64, 0, 0)),
+             ("interleaved_batches", sh.Layout(b, 1, b, 1, 0, 0)), ("planar", sh.Layout(1, n, 1, n, 1, 1)))
+    for tag, lay in cases:
+        ext = n * b if lay is None else (b - 1) * lay.in_distance + (n - 1) * lay.in_stride + 1
+        if lay is not None and lay.in_planar:
+            x = (torch.empty(ext, device=dev).uniform_(-1, 1), torch.empty(ext, device=dev).uniform_(-1, 1))
+            y = (torch.empty(ext, device=dev), torch.empty(ext, device=dev))
+        else:
+            x = torch.empty(2 * ext, device=dev).uniform_(-1, 1).view(torch.complex64)
+            y = torch.empty_like(x)
+        p = sh.Plan([n], b, sh.FORWARD, layout=lay)
+        run = (lambda: p.execute(x, y)) if lay is None else (lambda: p.execute_layout(x, y))
+        for _ in range(2):
+            run()
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            run()
+        b_.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b_) / reps
+        out[tag] = {"ms": round(ms, 4), "gbs": round(bytes_ / (ms * 1e-3) / 1e9, 1), "launches": p.launch_count(),
+                    "fused": lay is not None and "layout(" in p.describe()}
+        p.close()
+        del x, y
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -694,6 +732,7 @@ def main():
     # ---------------- multi-D configuration
     multid = None if args.no_sweep else multid_bench(sh, torch, dev)
     real = None if args.no_sweep else real_bench(sh, torch, dev)
+    layout = None if args.no_sweep else layout_bench(sh, torch, dev)
 
     # ---------------- precision configuration: plain FP32 VALU vs split-precision MFMA
     precision = None if args.no_precision else precision_bench(sh, torch, dev, lib)
@@ -760,6 +799,7 @@ def main():
         "fp32_hbm_sweep": sweep,
         "multid": multid,
         "real": real,
+        "layout": layout,
         "warm_cold": caches,
         "precision": precision,
         "work_queue": work_queue,

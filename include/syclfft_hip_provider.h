@@ -131,6 +131,43 @@ SYCLFFT_HIP_API int syclfft_hip_plan_create_ex(
 SYCLFFT_HIP_API int syclfft_hip_real_plan_create(
     const syclfft_hip_plan_config_v1 *config, void **plan, char *err, size_t cap);
 
+/* Caller data layout of a layout plan, FFTW's advanced-interface strides plus split-complex
+ * storage.  Element x (the row-major flat index over lengths[0..rank)) of transform b sits at
+ * b * distance + x * stride, counted in complex elements for interleaved storage and in real
+ * elements of each array for planar storage.  The reference's FFTW provider passes stride 1
+ * and distance prod(lengths) (src/providers/fftw/fftw_provider.cpp:45-60, istride/idist and
+ * ostride/odist of fftw_plan_many_dft); strides and split-complex storage are outside the
+ * reference's scope (README.md:18-20). */
+typedef struct syclfft_hip_layout_v1
+{
+    uint64_t in_stride;    /* >= 1 */
+    uint64_t in_distance;  /* >= 1 */
+    uint64_t out_stride;   /* >= 1 */
+    uint64_t out_distance; /* >= 1 */
+    uint32_t in_planar;    /* 1: input as separate real and imaginary arrays */
+    uint32_t out_planar;   /* 1: output as separate real and imaginary arrays */
+} syclfft_hip_layout_v1;
+
+/* Build a C2C plan over a caller layout.  The config is that of syclfft_hip_plan_create_ex;
+ * the layout must map distinct (transform, element) pairs to distinct elements (not
+ * checked).  A rank-1 interleaved layout whose transform is one tile launch is addressed
+ * directly by that launch (no extra memory traffic); every other layout runs a gather into
+ * plan scratch, the contiguous plan in place there, and a scatter, i.e. two more passes
+ * over the data.  Executed with syclfft_hip_layout_execute; syclfft_hip_plan_execute also
+ * accepts it when both sides are interleaved.  No reference counterpart (README.md:18-20). */
+SYCLFFT_HIP_API int syclfft_hip_layout_plan_create(
+    const syclfft_hip_plan_config_v1 *config, const syclfft_hip_layout_v1 *layout, void **plan, char *err,
+    size_t cap);
+
+/* Execute a layout plan: `in` / `out` are the interleaved arrays, or the real-part arrays of
+ * planar sides, whose imaginary-part arrays are in_imag / out_imag (NULL for interleaved
+ * sides).  In-place plans take in == out and in_imag == out_imag; out-of-place plans take
+ * non-overlapping input and output ranges.  Ordering, dependencies and `done` as for
+ * syclfft_hip_plan_execute. */
+SYCLFFT_HIP_API int syclfft_hip_layout_execute(
+    void *plan, const void *in, const void *in_imag, void *out, void *out_imag, void *stream, void *const *deps,
+    size_t ndeps, void *done, char *err, size_t cap);
+
 /* Free tables and scratch after the plan's last execution has finished
  * (replaces impl::~impl, src/sycl/sycl.cpp:593-613). */
 SYCLFFT_HIP_API void syclfft_hip_plan_destroy(void *plan);

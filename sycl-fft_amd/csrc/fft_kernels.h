@@ -51,5 +51,10 @@ extern "C" int sfh_launch_queue(
 extern "C" int sfh_launch_real_pass(
     int c2r, const void *in, void *out, const void *tw, uint64_t M, uint64_t batch, int is_double, double scale,
     hipStream_t stream);
+// strided / planar layout conversion of batch x n complex values (element x of transform b
+// at b * dist + x * stride; planar = separate real and imaginary arrays)
+extern "C" int sfh_launch_layout_copy(
+    const void *s_re, const void *s_im, int64_t s_stride, int64_t s_dist, int s_planar, void *d_re, void *d_im,
+    int64_t d_stride, int64_t d_dist, int d_planar, uint64_t n, uint64_t batch, int is_double, hipStream_t stream);
 extern "C" int sfh_launch_copy_scale(
     const void *in, void *out, uint64_t count, int is_double, double scale, hipStream_t stream);

@@ -68,6 +68,40 @@ __global__ __launch_bounds__(256) void c2r_tangle(const T *x, T *z, const T *tw,
         z[2 * i + 1] = scale * (ai + tr);
     }
 }
+// Layout conversion of `batch` transforms of `n` complex values each (element x of transform
+// b at b * dist + x * stride, in complex elements): interleaved (re, im) pairs at re[2 o],
+// re[2 o + 1], or planar (split-complex) storage with the real parts at re[o] and the
+// imaginary parts at im[o].  The grid's x dimension walks the elements of a transform and
+// its y dimension the transforms, so no index division is needed.
+template <typename T>
+__global__ __launch_bounds__(256) void layout_copy(
+    const T *s_re, const T *s_im, int64_t s_stride, int64_t s_dist, int s_planar, T *d_re, T *d_im,
+    int64_t d_stride, int64_t d_dist, int d_planar, uint64_t n, uint64_t batch)
+{
+    const uint64_t xstep = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t b = blockIdx.y; b < batch; b += gridDim.y) {
+        const int64_t sb = static_cast<int64_t>(b) * s_dist, db = static_cast<int64_t>(b) * d_dist;
+        for (uint64_t x = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; x < n; x += xstep) {
+            const int64_t so = sb + static_cast<int64_t>(x) * s_stride;
+            const int64_t dop = db + static_cast<int64_t>(x) * d_stride;
+            T vr, vi;
+            if (s_planar) {
+                vr = s_re[so];
+                vi = s_im[so];
+            } else {
+                vr = s_re[2 * so];
+                vi = s_re[2 * so + 1];
+            }
+            if (d_planar) {
+                d_re[dop] = vr;
+                d_im[dop] = vi;
+            } else {
+                d_re[2 * dop] = vr;
+                d_re[2 * dop + 1] = vi;
+            }
+        }
+    }
+}
 } // namespace
 
 extern "C" int sfh_launch_real_pass(
@@ -119,6 +153,27 @@ extern "C" int sfh_launch_copy_scale(
         hipLaunchKernelGGL(
             copy_scale<float>, dim3(blocks), dim3(256), 0, stream, static_cast<const float *>(in),
             static_cast<float *>(out), n2, static_cast<float>(scale));
+    }
+    return static_cast<int>(hipGetLastError());
+}
+
+extern "C" int sfh_launch_layout_copy(
+    const void *s_re, const void *s_im, int64_t s_stride, int64_t s_dist, int s_planar, void *d_re, void *d_im,
+    int64_t d_stride, int64_t d_dist, int d_planar, uint64_t n, uint64_t batch, int is_double, hipStream_t stream)
+{
+    if (n == 0 || batch == 0) {
+        return 0;
+    }
+    const uint64_t bx = (n + 255) / 256;
+    const dim3 grid(static_cast<unsigned>(bx < 4096 ? bx : 4096), static_cast<unsigned>(batch < 65535 ? batch : 65535));
+    if (is_double) {
+        hipLaunchKernelGGL(layout_copy<double>, grid, dim3(256), 0, stream, static_cast<const double *>(s_re),
+                           static_cast<const double *>(s_im), s_stride, s_dist, s_planar, static_cast<double *>(d_re),
+                           static_cast<double *>(d_im), d_stride, d_dist, d_planar, n, batch);
+    } else {
+        hipLaunchKernelGGL(layout_copy<float>, grid, dim3(256), 0, stream, static_cast<const float *>(s_re),
+                           static_cast<const float *>(s_im), s_stride, s_dist, s_planar, static_cast<float *>(d_re),
+                           static_cast<float *>(d_im), d_stride, d_dist, d_planar, n, batch);
     }
     return static_cast<int>(hipGetLastError());
 }

@@ -297,6 +297,14 @@ struct plan
     uint64_t real_m{};
     void *real_tw{};
     void *real_scratch{};
+    // layout plans (syclfft_hip_layout_plan_create): the caller's strided / planar layout over
+    // the contiguous plan `core`.  lay_fused: core's one launch addresses the caller's buffers
+    // with the layout's strides; otherwise gather into lay_scratch, core in place there, scatter.
+    int layout{};
+    syclfft_hip_layout_v1 lay{};
+    struct plan *core{};
+    bool lay_fused{};
+    void *lay_scratch{};
     // unit-table ring for executions over pointer arrays (allocated on first use): per slot
     // a device table block, its pinned host staging copy, and the event after which the
     // slot's last execution no longer reads it
@@ -1477,6 +1485,12 @@ static void release(plan *p)
     if (p->half) {
         release(p->half);
     }
+    if (p->core) {
+        release(p->core);
+    }
+    if (p->lay_scratch) {
+        (void)hipFree(p->lay_scratch);
+    }
     if (p->real_tw) {
         (void)hipFree(p->real_tw);
     }
@@ -1504,7 +1518,10 @@ static void release(plan *p)
     delete p;
 }
 
-static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune)
+// batch_inner: the batch is the innermost dimension (transform b's element x at x B + b, the
+// layout of stride B and distance 1), so every axis is planned as a strided axis whose inner
+// extent includes the batch, with tiles of adjacent transforms.
+static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune, bool batch_inner = false)
 {
     std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
     p->tune = tune;
@@ -1529,7 +1546,7 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     size_t first_begin = 0, first_end = 0, last_begin = 0;
     bool any = false;
     for (uint32_t axis = 0; axis < cfg->rank; ++axis) {
-        uint64_t inner = 1;
+        uint64_t inner = batch_inner ? cfg->batch_count : 1;
         for (uint32_t k = axis + 1; k < cfg->rank; ++k) {
             inner *= cfg->lengths[k];
         }
@@ -1989,15 +2006,14 @@ static void check_buffers(const plan &p, const void *in, const void *out)
 static void execute_real(
     plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done);
 
-static void execute_plan(
-    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
-    hipEvent_t done)
+static void execute_layout(
+    plan &p, const void *in, const void *in_im, void *out, void *out_im, hipStream_t stream, void *const *deps,
+    size_t ndeps, hipEvent_t done);
+
+// The launches of a complex plan over buffers already checked.
+static void execute_checked(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done)
 {
-    if (p.real) {
-        execute_real(p, in, out, stream, deps, ndeps, done);
-        return;
-    }
-    check_buffers(p, in, out);
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
     const bool capturing = stream_capturing(stream);
@@ -2033,6 +2049,26 @@ static void execute_plan(
     if (done) {
         hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
     }
+}
+
+static void execute_plan(
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps,
+    hipEvent_t done)
+{
+    if (p.real) {
+        execute_real(p, in, out, stream, deps, ndeps, done);
+        return;
+    }
+    if (p.layout) {
+        if (p.lay.in_planar || p.lay.out_planar) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                            "planar layout plans execute through syclfft_hip_layout_execute");
+        }
+        execute_layout(p, in, nullptr, out, nullptr, stream, deps, ndeps, done);
+        return;
+    }
+    check_buffers(p, in, out);
+    execute_checked(p, in, out, stream, deps, ndeps, done);
 }
 
 // Execution of `count` transforms of the plan's shape given by pointer arrays: transform k
@@ -2628,9 +2664,220 @@ static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
     return p.release();
 }
 
+
+// Elements of one side of a layout: the highest offset touched plus one.
+static uint64_t layout_extent(uint64_t batch, uint64_t transform, uint64_t stride, uint64_t dist)
+{
+    return (batch - 1) * dist + (transform - 1) * stride + 1;
+}
+
+// Layout plan: the contiguous plan of the config (`core`) plus the caller's strides and
+// planar flags.  A rank-1 interleaved layout whose core is one tile launch reading the input
+// and writing the output as contiguous columns (element j of transform b at b n + j) gets
+// the layout's distance and stride written into that launch's column and element strides,
+// so the launch reads and writes the caller's buffers directly; its tile, lane mapping and
+// LDS layout stay those of the contiguous plan.  Every other layout gathers into scratch,
+// runs the core in place there and scatters.
+static plan *create_layout_plan(const syclfft_hip_plan_config_v1 *cfg, const syclfft_hip_layout_v1 *lay)
+{
+    validate_config(cfg);
+    if (!lay) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "layout is null");
+    }
+    const uint64_t lim = uint64_t(1) << 40;
+    for (uint64_t v : {lay->in_stride, lay->in_distance, lay->out_stride, lay->out_distance}) {
+        if (v == 0 || v >= lim) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "layout strides and distances must be in [1, 2^40)");
+        }
+    }
+    if (lay->in_planar > 1 || lay->out_planar > 1) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "layout planar flags must be 0 or 1");
+    }
+    uint64_t T = 1;
+    for (uint32_t d = 0; d < cfg->rank; ++d) {
+        T *= cfg->lengths[d];
+    }
+    const uint64_t B = cfg->batch_count;
+    if (layout_extent(B, T, lay->in_stride, lay->in_distance) >= (uint64_t(1) << 62)
+        || layout_extent(B, T, lay->out_stride, lay->out_distance) >= (uint64_t(1) << 62)) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "layout extent exceeds 2^62 elements");
+    }
+    const bool same = lay->in_stride == lay->out_stride && lay->in_distance == lay->out_distance
+                      && lay->in_planar == lay->out_planar;
+    const bool interleaved = !lay->in_planar && !lay->out_planar;
+    std::unique_ptr<plan, void (*)(plan *)> core(nullptr, release);
+    bool fused = false;
+    if (interleaved && same && B > 1 && lay->in_stride == B && lay->in_distance == 1 && T > 1) {
+        // interleaved batches: the contiguous plan of the transposed problem, whose axes are
+        // strided by B with adjacent transforms in each tile, reads and writes the caller's
+        // buffers directly (every plan form, any rank)
+        std::string reason;
+        const int dev = check_device(cfg->device, reason);
+        if (dev < 0) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_PROVIDER_UNAVAILABLE, reason);
+        }
+        tuning t{};
+        if (const char *forced = std::getenv("SYCLFFT_HIP_TUNE")) {
+            (void)tuning::parse(forced, t);
+        }
+        core.reset(build_plan(cfg, dev, t, true));
+        for (auto &l : core->launches) {
+            l.desc += " layout(batch-inner " + std::to_string(B) + ")";
+        }
+        fused = true;
+    }
+    if (!fused && cfg->rank == 1 && interleaved && (!cfg->in_place || same) && T > 1) {
+        core.reset(create_plan(cfg));
+        plan &c = *core;
+        const uint64_t n = T;
+        if (!c.copy_only && c.launches.size() == 1) {
+            launch &l = c.launches[0];
+            sfh_tile_args &a = l.args;
+            const bool contiguous = l.src == BUF_IN && l.dst == BUF_OUT && l.src_off == 0 && l.dst_off == 0
+                                    && a.nblo == 1 && a.in_j == 1 && a.out_k == 1
+                                    && static_cast<uint64_t>(a.in_w) == n && static_cast<uint64_t>(a.out_w) == n
+                                    && a.in_units == nullptr && a.out_units == nullptr;
+            const auto span = [&](uint64_t sw, uint64_t sj) { return sw * (a.W - 1) + sj * (a.L - 1); };
+            const uint64_t i31 = uint64_t(1) << 31;
+            if (contiguous && lay->in_distance < i31 && lay->out_distance < i31 && lay->in_stride < i31
+                && lay->out_stride < i31 && span(lay->in_distance, lay->in_stride) < i31
+                && span(lay->out_distance, lay->out_stride) < i31) {
+                a.in_w = static_cast<int32_t>(lay->in_distance);
+                a.in_j = static_cast<int32_t>(lay->in_stride);
+                a.out_w = static_cast<int32_t>(lay->out_distance);
+                a.out_k = static_cast<int32_t>(lay->out_stride);
+                l.desc += " layout(in " + std::to_string(lay->in_stride) + "/" + std::to_string(lay->in_distance)
+                          + ", out " + std::to_string(lay->out_stride) + "/" + std::to_string(lay->out_distance) + ")";
+                fused = true;
+            }
+        }
+        if (!fused) {
+            core.reset();
+        }
+    }
+    if (!fused) {
+        syclfft_hip_plan_config_v1 cc = *cfg;
+        cc.in_place = 1;
+        core.reset(create_plan(&cc));
+    }
+    std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
+    p->cfg = *cfg;
+    p->device = core->device;
+    p->dbl = core->dbl;
+    p->esize = core->esize;
+    p->variants = core->variants;
+    p->nvariants = core->nvariants;
+    p->tune = core->tune;
+    p->transform = T;
+    p->total = T * B;
+    p->scale = core->scale;
+    p->layout = 1;
+    p->lay = *lay;
+    p->lay_fused = fused;
+    device_guard guard(p->device);
+    if (!fused) {
+        hip_check(hipMalloc(&p->lay_scratch, p->total * p->esize), "hipMalloc(layout scratch)");
+    }
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
+    p->core = core.release();
+    trace(std::string("layout plan ") + (fused ? "fused" : "gather-scatter") + " in " + std::to_string(lay->in_stride)
+          + "/" + std::to_string(lay->in_distance) + (lay->in_planar ? " planar" : "") + " out "
+          + std::to_string(lay->out_stride) + "/" + std::to_string(lay->out_distance)
+          + (lay->out_planar ? " planar" : ""));
+    return p.release();
+}
+
+static void execute_layout(
+    plan &p, const void *in, const void *in_im, void *out, void *out_im, hipStream_t stream, void *const *deps,
+    size_t ndeps, hipEvent_t done)
+{
+    const syclfft_hip_layout_v1 &L = p.lay;
+    const uint64_t B = p.cfg.batch_count, T = p.transform;
+    if (static_cast<bool>(in_im) != static_cast<bool>(L.in_planar)
+        || static_cast<bool>(out_im) != static_cast<bool>(L.out_planar)) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER,
+                        "imaginary-part pointers are required for planar sides and must be null otherwise");
+    }
+    struct range
+    {
+        uintptr_t a, b;
+    };
+    std::vector<range> ins, outs;
+    const auto side = [&](const void *re, const void *im, uint64_t stride, uint64_t dist, uint32_t planar,
+                          const char *name, const char *name_im, std::vector<range> &rs) {
+        const uint64_t bytes = layout_extent(B, T, stride, dist) * (planar ? p.esize / 2 : p.esize);
+        validate_pointer(p, re, bytes, name);
+        rs.push_back({reinterpret_cast<uintptr_t>(re), reinterpret_cast<uintptr_t>(re) + bytes});
+        if (planar) {
+            validate_pointer(p, im, bytes, name_im);
+            rs.push_back({reinterpret_cast<uintptr_t>(im), reinterpret_cast<uintptr_t>(im) + bytes});
+            if (rs[0].a < rs[1].b && rs[1].a < rs[0].b) {
+                throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                                std::string(name) + " real and imaginary arrays overlap");
+            }
+        }
+    };
+    side(in, in_im, L.in_stride, L.in_distance, L.in_planar, "input", "input imaginary part", ins);
+    side(out, out_im, L.out_stride, L.out_distance, L.out_planar, "output", "output imaginary part", outs);
+    if (p.cfg.in_place) {
+        if (in != out || in_im != out_im) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "In-place plan must be executed with execute(inout)");
+        }
+    } else {
+        for (const range &x : ins) {
+            for (const range &y : outs) {
+                if (x.a < y.b && y.a < x.b) {
+                    throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "Out-of-place input and output overlap");
+                }
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lock(p.mu);
+    device_guard guard(p.device);
+    const bool capturing = stream_capturing(stream);
+    for (size_t i = 0; i < ndeps; ++i) {
+        if (deps[i]) {
+            hip_check(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(deps[i]), 0), "hipStreamWaitEvent");
+        }
+    }
+    if (!capturing) {
+        wait_last(p, stream);
+    }
+    const int dbl = p.dbl ? 1 : 0;
+    if (p.lay_fused) {
+        execute_checked(*p.core, in, out, stream, nullptr, 0, nullptr);
+    } else {
+        const int64_t t = static_cast<int64_t>(T);
+        hip_check(static_cast<hipError_t>(sfh_launch_layout_copy(
+                      in, in_im, static_cast<int64_t>(L.in_stride), static_cast<int64_t>(L.in_distance),
+                      static_cast<int>(L.in_planar), p.lay_scratch, nullptr, 1, t, 0, T, B, dbl, stream)),
+                  "layout gather launch");
+        execute_checked(*p.core, p.lay_scratch, p.lay_scratch, stream, nullptr, 0, nullptr);
+        hip_check(static_cast<hipError_t>(sfh_launch_layout_copy(
+                      p.lay_scratch, nullptr, 1, t, 0, out, out_im, static_cast<int64_t>(L.out_stride),
+                      static_cast<int64_t>(L.out_distance), static_cast<int>(L.out_planar), T, B, dbl, stream)),
+                  "layout scatter launch");
+    }
+    if (!capturing) {
+        hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
+        set_last(p, stream);
+    }
+    if (done) {
+        hip_check(hipEventRecord(done, stream), "hipEventRecord(done)");
+    }
+}
+
 static std::string describe_plan(const plan &p)
 {
     std::string s;
+    if (p.layout) {
+        const std::string c = describe_plan(*p.core);
+        if (p.lay_fused) {
+            return c;
+        }
+        return "layout-gather\n" + c.substr(0, c.rfind("tuning")) + "layout-scatter\n" + "tuning " + p.core->tune.str()
+               + "\n";
+    }
     if (p.real) {
         const std::string pass = p.real == 1 ? "r2c-untangle" : "c2r-tangle";
         if (p.real == 2) {
@@ -2751,6 +2998,64 @@ extern "C" int syclfft_hip_real_plan_create(
     return SYCLFFT_HIP_STATUS_PLANNING_FAILED;
 }
 
+extern "C" int syclfft_hip_layout_plan_create(
+    const syclfft_hip_plan_config_v1 *config, const syclfft_hip_layout_v1 *layout, void **plan, char *err,
+    size_t cap)
+{
+    if (plan) {
+        *plan = nullptr;
+    }
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "plan output pointer is null");
+        }
+        *plan = sfh::create_layout_plan(config, layout);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::bad_alloc &) {
+        sfh::set_err(err, cap, "Unable to allocate plan memory");
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+    }
+    return SYCLFFT_HIP_STATUS_PLANNING_FAILED;
+}
+
+extern "C" int syclfft_hip_layout_execute(
+    void *plan, const void *in, const void *in_imag, void *out, void *out_imag, void *stream, void *const *deps,
+    size_t ndeps, void *done, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE, "plan handle is null");
+        }
+        auto &p = *static_cast<sfh::plan *>(plan);
+        if (!p.layout) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "not a layout plan (syclfft_hip_layout_plan_create)");
+        }
+        if (ndeps && !deps) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "dependency list is null");
+        }
+        sfh::execute_layout(p, in, in_imag, out, out_imag, static_cast<hipStream_t>(stream), deps, ndeps,
+                            static_cast<hipEvent_t>(done));
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
 extern "C" void syclfft_hip_plan_destroy(void *plan)
 {
     sfh::release(static_cast<sfh::plan *>(plan));
@@ -2764,6 +3069,9 @@ extern "C" uint64_t syclfft_hip_plan_scratch_bytes(const void *plan)
     const auto *p = static_cast<const sfh::plan *>(plan);
     if (p->real) {
         return p->cfg.batch_count * p->real_m * p->esize + syclfft_hip_plan_scratch_bytes(p->half);
+    }
+    if (p->layout) {
+        return (p->lay_fused ? 0 : p->total * p->esize) + syclfft_hip_plan_scratch_bytes(p->core);
     }
     return p->scratch_elems * p->esize;
 }
@@ -2919,6 +3227,9 @@ extern "C" uint32_t syclfft_hip_plan_launch_count(const void *plan)
     const auto *p = static_cast<const sfh::plan *>(plan);
     if (p->real) {
         return 1u + syclfft_hip_plan_launch_count(p->half);
+    }
+    if (p->layout) {
+        return (p->lay_fused ? 0u : 2u) + syclfft_hip_plan_launch_count(p->core);
     }
     return static_cast<uint32_t>(p->launches.size()) + (p->copy_only ? 1u : 0u);
 }

@@ -46,6 +46,8 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_plan_create",
     "syclfft_hip_plan_create_ex",
     "syclfft_hip_real_plan_create",
+    "syclfft_hip_layout_plan_create",
+    "syclfft_hip_layout_execute",
     "syclfft_hip_plan_destroy",
     "syclfft_hip_plan_scratch_bytes",
     "syclfft_hip_plan_execute",
@@ -70,6 +72,19 @@ class SyclfftError(RuntimeError):
         super().__init__(f"{code}: {message}")
         self.code = code
         self.message = message
+
+
+class Layout(ctypes.Structure):
+    """syclfft_hip_layout_v1: element x of transform b at b * distance + x * stride (complex
+    elements for interleaved sides, real elements of each array for planar sides)."""
+    _fields_ = [
+        ("in_stride", ctypes.c_uint64),
+        ("in_distance", ctypes.c_uint64),
+        ("out_stride", ctypes.c_uint64),
+        ("out_distance", ctypes.c_uint64),
+        ("in_planar", ctypes.c_uint32),
+        ("out_planar", ctypes.c_uint32),
+    ]
 
 
 class PlanConfig(ctypes.Structure):
@@ -133,6 +148,13 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.syclfft_hip_real_plan_create.argtypes = [ctypes.POINTER(PlanConfig), ctypes.POINTER(ctypes.c_void_p),
                                                  ctypes.c_char_p, ctypes.c_size_t]
     lib.syclfft_hip_real_plan_create.restype = ctypes.c_int
+    lib.syclfft_hip_layout_plan_create.argtypes = [ctypes.POINTER(PlanConfig), ctypes.POINTER(Layout),
+                                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_layout_plan_create.restype = ctypes.c_int
+    lib.syclfft_hip_layout_execute.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.syclfft_hip_layout_execute.restype = ctypes.c_int
     lib.syclfft_hip_plan_destroy.argtypes = [ctypes.c_void_p]
     lib.syclfft_hip_plan_destroy.restype = None
     lib.syclfft_hip_plan_scratch_bytes.argtypes = [ctypes.c_void_p]
@@ -407,10 +429,14 @@ class Plan:
 
     def __init__(self, lengths: Sequence[int], batch: int = 1, direction: int = FORWARD,
                  double: bool = False, in_place: bool = False, normalization: str = "none",
-                 precision: str = "valu", measure: bool = False, device: int = -1, real: bool = False):
+                 precision: str = "valu", measure: bool = False, device: int = -1, real: bool = False,
+                 layout: "Layout | None" = None):
         """real=True: a real 1-D plan (syclfft_hip_real_plan_create): FORWARD = R2C (N real ->
         N/2 + 1 complex per transform), BACKWARD = C2R; execute on float/double tensors for
-        the real side and complex tensors for the spectrum side."""
+        the real side and complex tensors for the spectrum side.
+
+        layout: a layout plan (syclfft_hip_layout_plan_create) over strided and / or planar
+        buffers; execute it with execute_layout."""
         self._lib = load()
         self.real = bool(real)
         self.lengths = tuple(int(n) for n in lengths)
@@ -420,13 +446,18 @@ class Plan:
         self.in_place = bool(in_place)
         self.normalization = normalization
         self.precision = precision
+        self.layout = layout
         cfg = make_config(lengths, batch, direction, double, in_place, normalization, precision,
                           measure, device)
         err = ctypes.create_string_buffer(1024)
         h = ctypes.c_void_p()
         self._h = None
-        create = self._lib.syclfft_hip_real_plan_create if real else self._lib.syclfft_hip_plan_create_ex
-        st = create(ctypes.byref(cfg), ctypes.byref(h), err, 1024)
+        if layout is not None:
+            st = self._lib.syclfft_hip_layout_plan_create(ctypes.byref(cfg), ctypes.byref(layout), ctypes.byref(h),
+                                                          err, 1024)
+        else:
+            create = self._lib.syclfft_hip_real_plan_create if real else self._lib.syclfft_hip_plan_create_ex
+            st = create(ctypes.byref(cfg), ctypes.byref(h), err, 1024)
         if st != 0:
             _raise(st, err)
         self._h = h
@@ -501,6 +532,37 @@ class Plan:
             self.handle, n, ia, oa, ctypes.c_void_p(stream), arr, len(dl), ctypes.c_void_p(done), err, 1024)
         if st != 0:
             _raise(st, err)
+
+    def execute_layout_ptr(self, x, x_im, y, y_im, stream: int = 0, deps=(), done: int = 0) -> None:
+        """Execute a layout plan over raw device pointers (x_im / y_im: imaginary-part arrays
+        of planar sides, 0 for interleaved sides)."""
+        dl = [int(d) for d in deps]
+        arr = (ctypes.c_void_p * max(1, len(dl)))(*dl) if dl else None
+        err = ctypes.create_string_buffer(1024)
+        st = self._lib.syclfft_hip_layout_execute(
+            self.handle, ctypes.c_void_p(int(x)), ctypes.c_void_p(int(x_im)), ctypes.c_void_p(int(y)),
+            ctypes.c_void_p(int(y_im)), ctypes.c_void_p(stream), arr, len(dl), ctypes.c_void_p(done), err, 1024)
+        if st != 0:
+            _raise(st, err)
+
+    def execute_layout(self, x, y=None, stream=None, deps=(), done=None) -> None:
+        """Execute a layout plan on torch tensors: an interleaved side is one complex tensor, a
+        planar side a (real, imaginary) pair of float / double tensors."""
+        import torch
+
+        y = x if y is None else y
+
+        def parts(v):
+            if isinstance(v, (tuple, list)):
+                return v[0].data_ptr(), v[1].data_ptr(), v[0].device
+            return v.data_ptr(), 0, v.device
+
+        xr, xi, dev = parts(x)
+        yr, yi, _ = parts(y)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        self.execute_layout_ptr(xr, xi, yr, yi, s.cuda_stream, [e.cuda_event for e in deps], 0)
+        if done is not None:
+            done.record(s)
 
     def execute_many(self, xs, ys=None, stream=None, deps=(), done=None) -> None:
         """Execute on lists of torch tensors, one transform of the plan's shape each."""

@@ -146,8 +146,171 @@ static bool run_real(const std::string &spec)
     return ok;
 }
 
+// Layout plans ('L' float, 'M' double):
+//   L:n0[xn1[xn2]]:batch:dir:norm:inplace:istride:idist:ostride:odist:iplanar:oplanar
+// The input layout's gaps hold a sentinel that must not leak into the result; the output
+// layout's gaps (out of place) must keep theirs.
+static bool run_layout(const std::string &spec)
+{
+    char prec = 0, dims[128] = {0};
+    unsigned long long batch = 1, is = 1, id = 1, os = 1, od = 1;
+    int dir = -1, norm = 0, inplace = 0, ip = 0, op = 0;
+    if (std::sscanf(spec.c_str(), "%c:%127[0-9x]:%llu:%d:%d:%d:%llu:%llu:%llu:%llu:%d:%d", &prec, dims, &batch, &dir,
+                    &norm, &inplace, &is, &id, &os, &od, &ip, &op) != 12) {
+        std::printf("FAIL %s bad case spec\n", spec.c_str());
+        return false;
+    }
+    const bool dbl = prec == 'M';
+    syclfft_hip_plan_config_v1 cfg{};
+    cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
+    cfg.scalar = dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
+    cfg.direction = dir;
+    uint64_t lens[3] = {1, 1, 1}, T = 1;
+    {
+        std::string d(dims);
+        size_t pos = 0;
+        while (cfg.rank < 3 && pos <= d.size()) {
+            const size_t x = d.find('x', pos);
+            lens[cfg.rank] = std::strtoull(d.substr(pos, x == std::string::npos ? std::string::npos : x - pos).c_str(),
+                                           nullptr, 10);
+            T *= lens[cfg.rank];
+            cfg.lengths[cfg.rank] = lens[cfg.rank];
+            ++cfg.rank;
+            if (x == std::string::npos) {
+                break;
+            }
+            pos = x + 1;
+        }
+    }
+    cfg.batch_count = batch;
+    cfg.in_place = static_cast<uint32_t>(inplace);
+    cfg.normalization = static_cast<uint32_t>(norm);
+    cfg.device = -1;
+    syclfft_hip_layout_v1 lay{is, id, os, od, static_cast<uint32_t>(ip), static_cast<uint32_t>(op)};
+    char err[1024];
+    void *plan = nullptr;
+    if (syclfft_hip_layout_plan_create(&cfg, &lay, &plan, err, sizeof(err)) != 0) {
+        std::printf("FAIL %s create: %s\n", spec.c_str(), err);
+        return false;
+    }
+    const double sentinel = 4096.0;
+    const uint64_t iext = (batch - 1) * id + (T - 1) * is + 1, oext = (batch - 1) * od + (T - 1) * os + 1;
+    // host images of each array in doubles: interleaved sides 2 per element, planar sides 1 per array
+    std::vector<double> in_re(ip ? iext : 2 * iext, sentinel), in_im(ip ? iext : 0, sentinel);
+    std::vector<double> x(2 * T * batch), ref(2 * T * batch);
+    std::mt19937_64 rng(20261300 + T + batch);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (uint64_t b = 0; b < batch; ++b) {
+        for (uint64_t e = 0; e < T; ++e) {
+            double re = u(rng), im = u(rng);
+            if (!dbl) {
+                re = static_cast<float>(re);
+                im = static_cast<float>(im);
+            }
+            x[2 * (b * T + e)] = re;
+            x[2 * (b * T + e) + 1] = im;
+            const uint64_t o = b * id + e * is;
+            if (ip) {
+                in_re[o] = re;
+                in_im[o] = im;
+            } else {
+                in_re[2 * o] = re;
+                in_re[2 * o + 1] = im;
+            }
+        }
+    }
+    oracle_fftn_ld(static_cast<int>(cfg.rank), lens, batch, x.data(), ref.data(), dir, norm);
+    const size_t rs = dbl ? 8 : 4;
+    const auto upload = [&](const std::vector<double> &h) -> void * {
+        void *d = nullptr;
+        if (h.empty()) {
+            return nullptr;
+        }
+        hipMalloc(&d, h.size() * rs);
+        for (size_t i = 0; i < h.size(); ++i) {
+            if (dbl) {
+                static_cast<double *>(d)[i] = h[i];
+            } else {
+                static_cast<float *>(d)[i] = static_cast<float>(h[i]);
+            }
+        }
+        return d;
+    };
+    void *d_in = upload(in_re), *d_in_im = upload(in_im);
+    void *d_out = d_in, *d_out_im = d_in_im;
+    if (!inplace) {
+        d_out = upload(std::vector<double>(op ? oext : 2 * oext, sentinel));
+        d_out_im = op ? upload(std::vector<double>(oext, sentinel)) : nullptr;
+    }
+    if (syclfft_hip_layout_execute(plan, d_in, d_in_im, d_out, d_out_im, nullptr, nullptr, 0, nullptr, err,
+                                   sizeof(err))
+        != 0) {
+        std::printf("FAIL %s execute: %s\n", spec.c_str(), err);
+        return false;
+    }
+    const auto val = [&](void *d, uint64_t i) {
+        return dbl ? static_cast<double *>(d)[i] : static_cast<double>(static_cast<float *>(d)[i]);
+    };
+    double num = 0, den = 0;
+    std::vector<char> touched(op ? oext : 2 * oext, 0);
+    for (uint64_t b = 0; b < batch; ++b) {
+        for (uint64_t e = 0; e < T; ++e) {
+            const uint64_t o = b * od + e * os;
+            const double gr = op ? val(d_out, o) : val(d_out, 2 * o);
+            const double gi = op ? val(d_out_im, o) : val(d_out, 2 * o + 1);
+            const double er = ref[2 * (b * T + e)], ei = ref[2 * (b * T + e) + 1];
+            num += (gr - er) * (gr - er) + (gi - ei) * (gi - ei);
+            den += er * er + ei * ei;
+            if (op) {
+                touched[o] = 1;
+            } else {
+                touched[2 * o] = touched[2 * o + 1] = 1;
+            }
+        }
+    }
+    bool gaps_ok = true;
+    if (!inplace) {
+        for (size_t i = 0; i < touched.size(); ++i) {
+            if (!touched[i] && (val(d_out, i) != sentinel || (op && val(d_out_im, i) != sentinel))) {
+                gaps_ok = false;
+            }
+        }
+    }
+    const double rel = std::sqrt(num / (den > 0 ? den : 1));
+    const double u_ = dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(T))) * u_;
+    char desc[2048];
+    syclfft_hip_plan_describe(plan, desc, sizeof(desc));
+    for (char *c = desc; *c; ++c) {
+        if (*c == '\n') {
+            *c = '|';
+        }
+    }
+    const bool ok = rel <= tol && std::isfinite(rel) && gaps_ok;
+    std::printf("%s %s relL2=%.3e tol=%.1e gaps=%s launches=%u [%s]\n", ok ? "ok" : "FAIL", spec.c_str(), rel, tol,
+                gaps_ok ? "kept" : "WRITTEN", syclfft_hip_plan_launch_count(plan), desc);
+    std::fflush(stdout);
+    syclfft_hip_plan_destroy(plan);
+    for (void *d : {d_in, d_in_im}) {
+        if (d) {
+            hipFree(d);
+        }
+    }
+    if (!inplace) {
+        for (void *d : {d_out, d_out_im}) {
+            if (d) {
+                hipFree(d);
+            }
+        }
+    }
+    return ok;
+}
+
 static bool run_case(const std::string &spec)
 {
+    if (!spec.empty() && (spec[0] == 'L' || spec[0] == 'M')) {
+        return run_layout(spec);
+    }
     if (!spec.empty() && (spec[0] == 'R' || spec[0] == 'Q')) {
         return run_real(spec);
     }

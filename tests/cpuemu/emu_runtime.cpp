@@ -152,10 +152,14 @@ void emu_launch(dim3 grid, dim3 block, const std::function<void()> &body)
             threadIdx = dim3(t);
             blockDim = block;
             gridDim = grid;
-            for (unsigned b = 0; b < grid.x; ++b) {
-                blockIdx = dim3(b);
-                body();
-                bar.arrive_and_wait();
+            for (unsigned bz = 0; bz < grid.z; ++bz) {
+                for (unsigned by = 0; by < grid.y; ++by) {
+                    for (unsigned bx = 0; bx < grid.x; ++bx) {
+                        blockIdx = dim3(bx, by, bz);
+                        body();
+                        bar.arrive_and_wait();
+                    }
+                }
             }
         });
     }

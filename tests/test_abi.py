@@ -101,6 +101,38 @@ def test_invalid_real_configs_rejected(mutate, status, text):
     assert text.lower() in err.value.decode().lower()
 
 
+@pytest.mark.parametrize("field,value,text", [
+    ("in_stride", 0, "strides"),
+    ("out_distance", 0, "strides"),
+    ("in_distance", 1 << 40, "strides"),
+    ("in_planar", 2, "planar"),
+])
+def test_invalid_layouts_rejected(field, value, text):
+    """Layout plans: strides and distances in [1, 2^40), planar flags 0 / 1; checked before
+    any device work (INVALID_ARGUMENT)."""
+    lib = sh.load()
+    cfg = sh.make_config([8], 2)
+    lay = sh.Layout(1, 8, 1, 8, 0, 0)
+    setattr(lay, field, value)
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(256)
+    st = lib.syclfft_hip_layout_plan_create(ctypes.byref(cfg), ctypes.byref(lay), ctypes.byref(h), err, 256)
+    if h.value:
+        lib.syclfft_hip_plan_destroy(h)
+    assert st == 1, err.value
+    assert text in err.value.decode().lower()
+
+
+def test_layout_null_arguments():
+    lib = sh.load()
+    err = ctypes.create_string_buffer(64)
+    cfg = sh.make_config([8], 1)
+    h = ctypes.c_void_p()
+    assert lib.syclfft_hip_layout_plan_create(ctypes.byref(cfg), None, ctypes.byref(h), err, 64) == 1
+    assert b"layout is null" in err.value and not h.value
+    assert lib.syclfft_hip_layout_execute(None, None, None, None, None, None, None, 0, None, err, 64) == 2
+
+
 def test_null_arguments():
     lib = sh.load()
     err = ctypes.create_string_buffer(64)

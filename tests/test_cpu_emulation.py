@@ -160,3 +160,18 @@ def test_emulated_real_plans():
     _run(["R:2:3:-1:0:0", "R:16:4:-1:1:0", "R:4096:2:-1:0:0", "Q:2520:2:-1:3:0", "R:30:3:1:0:0",
           "R:4096:2:1:2:0", "Q:1000:2:1:3:0", "R:20014:1:-1:0:0", "Q:2:2:1:0:0", "R:65536:1:1:1:0"])
 
+
+def test_emulated_layout_plans():
+    """Layout plans ('L' float, 'M' double; ...:istride:idist:ostride:odist:iplanar:oplanar)
+    against the FP64 oracle over the gathered data, with sentinels in the layouts' gaps:
+    rank-1 interleaved layouts addressed by the one tile launch (single tile, in place,
+    Bluestein fused, N = 16384), interleaved batches planned as strided axes (four-step,
+    Bluestein fused and three-launch, 2-D, 3-D), and the gather / in-place core / scatter
+    path (planar sides, 2-D, a three-launch Bluestein core)."""
+    _run(["L:256:4:-1:0:0:2:600:3:900:0:0", "L:64:8:1:1:0:8:1:8:1:0:0", "M:120:3:-1:3:1:2:300:2:300:0:0",
+          "L:257:3:1:2:0:3:800:1:300:0:0", "L:16384:1:1:1:0:2:40000:1:16384:0:0",
+          "L:4096:2:-1:0:0:1:4096:1:4096:1:0", "M:2520:2:1:3:0:1:2600:2:5100:0:1", "L:1000:3:-1:2:1:1:1000:1:1000:1:1",
+          "M:12x10:2:-1:0:0:3:400:1:120:0:0", "L:64:4:1:1:0:5:400:1:64:1:1", "M:10007:1:-1:3:1:2:20014:2:20014:0:0",
+          # interleaved batches (stride = batch, distance 1): strided-axis plans over the caller's buffers
+          "M:12x10:4:-1:0:1:4:1:4:1:0:0", "L:16384:4:1:0:0:4:1:4:1:0:0", "L:257:3:-1:2:0:3:1:3:1:0:0",
+          "M:10007:2:1:3:0:2:1:2:1:0:0", "L:5x7x9:3:1:1:1:3:1:3:1:0:0"])

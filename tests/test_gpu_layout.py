@@ -1,0 +1,224 @@
+"""Layout plans (syclfft_hip_layout_plan_create / syclfft_hip_layout_execute) on the GPU.
+
+Element x of transform b sits at b * distance + x * stride on each side, interleaved or
+planar (separate real and imaginary arrays).  Each case fills the layout's gaps with a
+sentinel, runs the plan through the C ABI, and checks
+  * the result against the FP64 oracle over the gathered input, within c log2(N) u;
+  * that no gap of the output layout was written (out of place);
+  * bit-equality with the contiguous plan of the same config on the same values: the fused
+    path runs that plan's own launch with other strides, and the gather / scatter path
+    copies exactly around the contiguous in-place plan.
+Interleaved batches (stride = batch, distance 1, both sides) are planned as strided axes over
+the caller's buffers (any rank and plan form; checked against the oracle only, since their
+plan forms differ from the contiguous plan's); other rank-1 interleaved layouts over one
+tile launch are fused into it; the rest add the gather and scatter launches."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import oracle_fftn, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import syclfft_hip as sh  # noqa: E402
+
+U = {False: 2.0 ** -24, True: 2.0 ** -53}
+SENTINEL = 4096.0
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def side_image(vals, batch, T, stride, dist, planar, double):
+    """Host image of one side holding vals (batch x T complex) in the layout, gaps = sentinel."""
+    ext = (batch - 1) * dist + (T - 1) * stride + 1
+    rt = np.float64 if double else np.float32
+    off = (np.arange(batch)[:, None] * dist + np.arange(T)[None, :] * stride).reshape(-1)
+    if planar:
+        re = np.full(ext, SENTINEL, dtype=rt)
+        im = np.full(ext, SENTINEL, dtype=rt)
+        if vals is not None:
+            re[off] = vals.real.reshape(-1)
+            im[off] = vals.imag.reshape(-1)
+        return (re, im), off, ext
+    ct = np.complex128 if double else np.complex64
+    arr = np.full(ext, SENTINEL + 1j * SENTINEL, dtype=ct)
+    if vals is not None:
+        arr[off] = vals.reshape(-1)
+    return arr, off, ext
+
+
+def to_dev(img):
+    if isinstance(img, tuple):
+        return tuple(torch.from_numpy(a).to("cuda") for a in img)
+    return torch.from_numpy(img).to("cuda")
+
+
+def from_dev(d, off):
+    if isinstance(d, tuple):
+        re, im = (t.cpu().numpy() for t in d)
+        return re[off] + 1j * im[off], (re, im)
+    a = d.cpu().numpy()
+    return a[off], a
+
+
+def gaps_kept(host, off):
+    mask = np.ones(host[0].shape[0] if isinstance(host, tuple) else host.shape[0], dtype=bool)
+    mask[off] = False
+    if isinstance(host, tuple):
+        return all(bool(np.all(h[mask] == SENTINEL)) for h in host)
+    return bool(np.all(host[mask] == SENTINEL + 1j * SENTINEL))
+
+
+def contiguous_result(lengths, batch, direction, double, norm, x, in_place):
+    dt = torch.complex128 if double else torch.complex64
+    xd = torch.from_numpy(x.reshape(-1).astype(np.complex128 if double else np.complex64)).to("cuda")
+    with sh.Plan(lengths, batch, direction, double=double, in_place=in_place, normalization=norm) as p:
+        if in_place:
+            p.execute(xd)
+            yd = xd
+        else:
+            yd = torch.empty_like(xd, dtype=dt)
+            p.execute(xd, yd)
+        torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+# (lengths, batch, (istride, idist, ostride, odist, iplanar, oplanar), in_place, fusable)
+def _cases():
+    c = []
+    for n, b in [(64, 8), (256, 4), (4096, 3), (10007, 2), (257, 5), (2520, 4), (16384, 2)]:
+        c.append(([n], b, (2, 2 * n + 5, 3, 3 * n + 1, 0, 0), False, True))
+        c.append(([n], b, (b, 1, b, 1, 0, 0), False, True))            # interleaved batches
+        c.append(([n], b, (1, n + 7, 1, n + 7, 0, 0), True, True))      # in place, padded rows
+        c.append(([n], b, (1, n, 2, 2 * n + 1, 1, 0), False, False))    # planar in
+        c.append(([n], b, (3, 3 * n, 1, n, 0, 1), False, False))        # planar out
+    c.append(([1 << 20], 1, (2, 1 << 21, 1, 1 << 20, 0, 0), False, False))  # four-step core
+    c.append(([12, 10], 3, (3, 400, 1, 120, 0, 0), False, False))            # 2-D
+    c.append(([32, 32, 32], 2, (1, 32768, 1, 32768, 1, 1), True, False))     # 3-D planar in place
+    c.append(([1000], 3, (1, 1000, 1, 1000, 1, 1), True, False))
+    c.append(([12, 10], 4, (4, 1, 4, 1, 0, 0), True, True))                  # 2-D interleaved batches
+    c.append(([1 << 16], 8, (8, 1, 8, 1, 0, 0), False, True))                # four-step, batch inner
+    c.append(([5, 7, 9], 3, (3, 1, 3, 1, 0, 0), False, True))
+    return c
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("double", [False, True])
+@pytest.mark.parametrize("lengths,batch,lay,in_place,fused", CASES,
+                         ids=[f"{'x'.join(map(str, c[0]))}b{c[1]}-{'-'.join(map(str, c[2]))}-ip{int(c[3])}" for c in CASES])
+def test_layout_vs_oracle(lengths, batch, lay, in_place, fused, double):
+    say(f"case layout lengths={lengths} batch={batch} layout={lay} in_place={in_place} double={double}")
+    T = int(np.prod(lengths))
+    istride, idist, ostride, odist, ip, op = lay
+    g = np.random.default_rng(20261400 + T + batch)
+    ct = np.complex128 if double else np.complex64
+    x = (g.uniform(-1, 1, (batch, T)) + 1j * g.uniform(-1, 1, (batch, T))).astype(ct)
+    direction = sh.FORWARD if (T + batch) % 2 else sh.BACKWARD
+    norm = ["none", "orthogonal", "forward", "backward"][(T // 7 + batch) % 4]
+    L = sh.Layout(istride, idist, ostride, odist, ip, op)
+    with sh.Plan(lengths, batch, direction, double=double, in_place=in_place, normalization=norm, layout=L) as p:
+        desc = p.describe().strip()
+        is_fused = "layout(" in desc
+        img_in, ioff, _ = side_image(x, batch, T, istride, idist, ip, double)
+        d_in = to_dev(img_in)
+        if in_place:
+            d_out, ooff = d_in, ioff
+        else:
+            img_out, ooff, _ = side_image(None, batch, T, ostride, odist, op, double)
+            d_out = to_dev(img_out)
+        p.execute_layout(d_in, d_out)
+        torch.cuda.synchronize()
+        got, host = from_dev(d_out, ooff)
+    say(f"  {desc}")
+    if fused:
+        # rank-1 interleaved over a one-launch plan: fused wherever the contiguous plan is one launch
+        assert is_fused or "layout-gather" in desc
+    else:
+        assert not is_fused
+    got = got.reshape(batch, T)
+    ref = oracle_fftn(x.astype(np.complex128), lengths, batch, direction, norm).reshape(batch, T)
+    err = rel_l2(got, ref)
+    tol = 4 * max(1.0, np.log2(T)) * U[double]
+    say(f"  fused={is_fused} rel-L2 {err:.3e} (tol {tol:.1e})")
+    assert err <= tol
+    if not in_place:
+        assert gaps_kept(host, ooff), "a gap of the output layout was written"
+    if "batch-inner" in desc:
+        return  # planned as strided axes: other plan forms than the contiguous plan's
+    # the fused path runs the contiguous plan's launch; the gather path its in-place plan
+    contiguous = contiguous_result(lengths, batch, direction, double, norm, x, in_place or not is_fused)
+    assert np.array_equal(got.reshape(-1), contiguous), "differs from the contiguous plan"
+
+
+@pytest.mark.parametrize("n,batch,lay,fused", [
+    (4096, 4, (2, 8193, 2, 8193, 0, 0), True),
+    (10007, 2, (1, 10007, 1, 10007, 1, 0), False),
+    (10007, 16, (16, 1, 16, 1, 0, 0), True),
+])
+def test_layout_launch_count(n, batch, lay, fused):
+    """Fused layouts add no launch; gather / scatter layouts add two."""
+    L = sh.Layout(*lay)
+    with sh.Plan([n], batch, sh.FORWARD, layout=L) as p, sh.Plan([n], batch, sh.FORWARD) as c:
+        extra = p._lib.syclfft_hip_plan_launch_count(p.handle) - c._lib.syclfft_hip_plan_launch_count(c.handle)
+        say(f"case launch count n={n} layout={lay}: +{extra} [{p.describe().strip()}]")
+        assert extra == (0 if fused else 2)
+        assert ("layout(" in p.describe()) == fused
+
+
+def test_layout_errors():
+    say("case layout errors")
+    n, b = 256, 2
+    planar = sh.Plan([n], b, sh.FORWARD, layout=sh.Layout(1, n, 1, n, 1, 0))
+    inter = sh.Plan([n], b, sh.FORWARD, layout=sh.Layout(2, 2 * n, 2, 2 * n, 0, 0))
+    try:
+        re = torch.zeros(b * n, device="cuda")
+        im = torch.zeros(b * n, device="cuda")
+        y = torch.empty(b * n, dtype=torch.complex64, device="cuda")
+        # a planar plan through the interleaved entry
+        with pytest.raises(sh.SyclfftError) as e:
+            planar.execute(re, y)
+        assert e.value.code == "invalid_argument"
+        # the imaginary-part array is missing
+        with pytest.raises(sh.SyclfftError) as e:
+            planar.execute_layout_ptr(re.data_ptr(), 0, y.data_ptr(), 0)
+        assert e.value.code == "invalid_pointer"
+        # an interleaved side given an imaginary-part array
+        with pytest.raises(sh.SyclfftError) as e:
+            planar.execute_layout_ptr(re.data_ptr(), im.data_ptr(), y.data_ptr(), im.data_ptr())
+        assert e.value.code == "invalid_pointer"
+        # an input allocation shorter than the strided extent (2 (b - 1) n + 2 (n - 1) + 1
+        # elements): allocated with hipMalloc at exactly b n elements, since the caching
+        # allocator's segments are larger than the tensors carved from them
+        # by soname: the HIP runtime already loaded with torch, which the provider binds to too
+        hip = sh.ctypes.CDLL("libamdhip64.so.7")
+        short = sh.ctypes.c_void_p()
+        assert hip.hipMalloc(sh.ctypes.byref(short), sh.ctypes.c_size_t(b * n * 8)) == 0
+        try:
+            y2 = torch.empty(4 * b * n, dtype=torch.complex64, device="cuda")
+            with pytest.raises(sh.SyclfftError) as e:
+                inter.execute_layout_ptr(short.value, 0, y2.data_ptr(), 0)
+            assert e.value.code == "invalid_pointer"
+        finally:
+            hip.hipFree(short)
+        x = torch.zeros(b * n, dtype=torch.complex64, device="cuda")
+        # overlapping out-of-place ranges
+        big = torch.zeros(8 * b * n, dtype=torch.complex64, device="cuda")
+        with pytest.raises(sh.SyclfftError) as e:
+            inter.execute_layout_ptr(big.data_ptr(), 0, big.data_ptr() + 8, 0)
+        assert e.value.code == "invalid_argument"
+        # not a layout plan
+        with sh.Plan([n], b, sh.FORWARD) as plain:
+            err = sh.ctypes.create_string_buffer(256)
+            st = plain._lib.syclfft_hip_layout_execute(plain.handle, x.data_ptr(), None, y.data_ptr(), None, None,
+                                                       None, 0, None, err, 256)
+            assert st == 1 and b"not a layout plan" in err.value
+    finally:
+        planar.close()
+        inter.close()
