@@ -470,19 +470,25 @@ def queue_bench(sh, torch, dev, nplans=256, reps=20, seed=20260007):
 
 
 def real_bench(sh, torch, dev, reps=10):
-    """Real 1-D plans: R2C and C2R, fp32 N=4096 batch 16384 and fp64 N=4096 batch 8192
-    (256 MiB of real values), time per execute and algorithmic GB/s (real side N values,
-    complex side N/2 + 1 values, one read and one write)."""
+    """Real plans: R2C and C2R, fp32 N=4096 batch 16384 and fp64 N=4096 batch 8192 (256 MiB
+    of real values), and 2-D fp32 2048 x 2048 batch 16 (the last axis as real rows, then the
+    leading axis over the half spectrum); time per execute and algorithmic GB/s (real side
+    prod(n) values, complex side prod(n) / N (N/2 + 1) values, one read and one write)."""
     out = {}
     stream = torch.cuda.current_stream(dev)
-    for name, n, b, dbl in (("f32_4096x16384", 4096, 16384, False), ("f64_4096x8192", 4096, 8192, True)):
+    for name, lengths, b, dbl in (("f32_4096x16384", [4096], 16384, False), ("f64_4096x8192", [4096], 8192, True),
+                                  ("f32_2d_2048x2048x16", [2048, 2048], 16, False)):
         rdt, cdt = (torch.float64, torch.complex128) if dbl else (torch.float32, torch.complex64)
-        x = torch.empty(n * b, dtype=rdt, device=dev).uniform_(-1, 1)
-        y = torch.empty(b * (n // 2 + 1), dtype=cdt, device=dev)
+        T = 1
+        for v in lengths:
+            T *= v
+        n = lengths[-1]
+        x = torch.empty(T * b, dtype=rdt, device=dev).uniform_(-1, 1)
+        y = torch.empty(b * T // n * (n // 2 + 1), dtype=cdt, device=dev)
         z = torch.empty_like(x)
-        bytes_ = n * b * x.element_size() + y.numel() * y.element_size()
+        bytes_ = T * b * x.element_size() + y.numel() * y.element_size()
         for direction, src, dst, tag in ((sh.FORWARD, x, y, "r2c"), (sh.BACKWARD, y, z, "c2r")):
-            p = sh.Plan([n], b, direction, double=dbl, real=True, measure=True)
+            p = sh.Plan(lengths, b, direction, double=dbl, real=True, measure=True)
             for _ in range(2):
                 p.execute(src, dst)
             a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -1521,7 +1521,11 @@ static void release(plan *p)
 // batch_inner: the batch is the innermost dimension (transform b's element x at x B + b, the
 // layout of stride B and distance 1), so every axis is planned as a strided axis whose inner
 // extent includes the batch, with tiles of adjacent transforms.
-static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune, bool batch_inner = false)
+// trail > 1: every transform carries an untransformed innermost dimension of that extent
+// (C order [batch][lengths...][trail]); the axes are strided by it.  Multi-D real plans use
+// it for the leading axes of the half spectrum [..][n/2 + 1].
+static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune, bool batch_inner = false,
+                        uint64_t trail = 1)
 {
     std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
     p->tune = tune;
@@ -1539,14 +1543,14 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     for (uint32_t i = 0; i < cfg->rank; ++i) {
         p->transform *= cfg->lengths[i];
     }
-    p->total = p->transform * cfg->batch_count;
+    p->total = p->transform * cfg->batch_count * trail;
     p->scale = norm_scale(cfg->normalization, cfg->direction, p->transform);
 
     // axis loop in the reference order (src/sycl/sycl.cpp:692-767)
     size_t first_begin = 0, first_end = 0, last_begin = 0;
     bool any = false;
     for (uint32_t axis = 0; axis < cfg->rank; ++axis) {
-        uint64_t inner = batch_inner ? cfg->batch_count : 1;
+        uint64_t inner = (batch_inner ? cfg->batch_count : 1) * trail;
         for (uint32_t k = axis + 1; k < cfg->rank; ++k) {
             inner *= cfg->lengths[k];
         }
@@ -2564,7 +2568,8 @@ static void execute_queue(plan *const *plans, const void *const *ins, void *cons
 static void execute_real(
     plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done)
 {
-    const uint64_t B = p.cfg.batch_count, M = p.real_m;
+    const uint64_t rows = p.transform / (2 * p.real_m); // 1 for 1-D plans
+    const uint64_t B = p.cfg.batch_count * rows, M = p.real_m;
     const uint64_t real_bytes = B * M * p.esize;       // B N real values
     const uint64_t half_bytes = B * (M + 1) * p.esize; // B (M + 1) complex values
     const uint64_t in_bytes = p.real == 1 ? real_bytes : half_bytes;
@@ -2589,7 +2594,15 @@ static void execute_real(
     if (!capturing) {
         wait_last(p, stream);
     }
-    if (p.real == 1) {
+    if (p.core) { // multi-D: the last axis as 1-D real rows plus the leading axes
+        if (p.real == 1) {
+            execute_plan(*p.half, in, out, stream, nullptr, 0, nullptr);
+            execute_checked(*p.core, out, out, stream, nullptr, 0, nullptr);
+        } else {
+            execute_checked(*p.core, in, p.real_scratch, stream, nullptr, 0, nullptr);
+            execute_plan(*p.half, p.real_scratch, out, stream, nullptr, 0, nullptr);
+        }
+    } else if (p.real == 1) {
         execute_plan(*p.half, in, p.real_scratch, stream, nullptr, 0, nullptr);
         hip_check(static_cast<hipError_t>(sfh_launch_real_pass(0, p.real_scratch, out, p.real_tw, M, B, p.dbl ? 1 : 0,
                                                                p.scale, stream)),
@@ -2609,11 +2622,13 @@ static void execute_real(
     }
 }
 
+static plan *create_real_plan_nd(const syclfft_hip_plan_config_v1 *cfg);
+
 static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
 {
     validate_config(cfg);
-    if (cfg->rank != 1 || cfg->lengths[0] < 2 || cfg->lengths[0] % 2 != 0) {
-        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "real plans are 1-D with an even length >= 2");
+    if (cfg->lengths[cfg->rank - 1] < 2 || cfg->lengths[cfg->rank - 1] % 2 != 0) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "real plans need an even last length >= 2");
     }
     if (cfg->in_place) {
         throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
@@ -2621,6 +2636,9 @@ static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
     }
     if (cfg->precision != SYCLFFT_HIP_PRECISION_VALU) {
         throw sfh_error(SYCLFFT_HIP_STATUS_UNSUPPORTED_PRECISION, "real plans use the VALU precision mode");
+    }
+    if (cfg->rank > 1) {
+        return create_real_plan_nd(cfg);
     }
     syclfft_hip_plan_config_v1 h = *cfg;
     h.lengths[0] = cfg->lengths[0] / 2;
@@ -2867,6 +2885,66 @@ static void execute_layout(
     }
 }
 
+
+// Multi-D real plan, lengths n_0 .. n_{r-1} (n_{r-1} = N even), half spectrum h = N/2 + 1 on
+// the last axis, C order [batch][n_0]..[n_{r-2}][h]:
+//   R2C: the 1-D real plan of length N over batch * n_0 ... n_{r-2} rows, writing the half
+//        spectrum; then the complex plan over axes 0 .. r-2 in place on it, with h as an
+//        untransformed innermost extent (every axis strided by h, tiles of adjacent bins);
+//   C2R: that complex plan (backward) from the input into scratch, then the 1-D C2R plan.
+// Each part normalizes by its own lengths with the plan's mode, which multiply to the
+// normalization by prod(n).
+static plan *create_real_plan_nd(const syclfft_hip_plan_config_v1 *cfg)
+{
+    const uint32_t r = cfg->rank;
+    const uint64_t N = cfg->lengths[r - 1], hb = N / 2 + 1;
+    uint64_t rows = 1;
+    for (uint32_t d = 0; d + 1 < r; ++d) {
+        rows *= cfg->lengths[d];
+    }
+    syclfft_hip_plan_config_v1 c1 = *cfg; // the last axis, as 1-D real rows
+    c1.rank = 1;
+    c1.lengths[0] = N;
+    c1.lengths[1] = c1.lengths[2] = 0;
+    c1.batch_count = cfg->batch_count * rows;
+    std::unique_ptr<plan, void (*)(plan *)> last(create_real_plan(&c1), release);
+    syclfft_hip_plan_config_v1 cr = *cfg; // the leading axes of the half spectrum
+    cr.rank = r - 1;
+    for (uint32_t d = 0; d < 3; ++d) {
+        cr.lengths[d] = d + 1 < r ? cfg->lengths[d] : 0;
+    }
+    cr.in_place = cfg->direction == -1 ? 1u : 0u;
+    cr.measure = 0;
+    tuning t{};
+    if (const char *forced = std::getenv("SYCLFFT_HIP_TUNE")) {
+        (void)tuning::parse(forced, t);
+    }
+    std::unique_ptr<plan, void (*)(plan *)> lead(build_plan(&cr, last->device, t, false, hb), release);
+    std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
+    p->cfg = *cfg;
+    p->device = last->device;
+    p->dbl = last->dbl;
+    p->esize = last->esize;
+    p->variants = last->variants;
+    p->nvariants = last->nvariants;
+    p->tune = last->tune;
+    p->transform = rows * N;
+    p->total = p->transform * cfg->batch_count;
+    p->scale = norm_scale(cfg->normalization, cfg->direction, p->transform);
+    p->real = cfg->direction == -1 ? 1 : 2;
+    p->real_m = N / 2;
+    device_guard guard(p->device);
+    if (p->real == 2) {
+        hip_check(hipMalloc(&p->real_scratch, cfg->batch_count * rows * hb * p->esize), "hipMalloc(real scratch)");
+    }
+    hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
+    p->half = last.release();
+    p->core = lead.release();
+    trace(std::string(p->real == 1 ? "real r2c" : "real c2r") + " rank " + std::to_string(r) + " last axis n="
+          + std::to_string(N));
+    return p.release();
+}
+
 static std::string describe_plan(const plan &p)
 {
     std::string s;
@@ -2877,6 +2955,11 @@ static std::string describe_plan(const plan &p)
         }
         return "layout-gather\n" + c.substr(0, c.rfind("tuning")) + "layout-scatter\n" + "tuning " + p.core->tune.str()
                + "\n";
+    }
+    if (p.real && p.core) {
+        const std::string a = describe_plan(*p.half), b = describe_plan(*p.core);
+        const std::string a0 = a.substr(0, a.rfind("tuning")), b0 = b.substr(0, b.rfind("tuning"));
+        return (p.real == 1 ? a0 + b0 : b0 + a0) + "tuning " + p.half->tune.str() + "\n";
     }
     if (p.real) {
         const std::string pass = p.real == 1 ? "r2c-untangle" : "c2r-tangle";
@@ -3067,6 +3150,10 @@ extern "C" uint64_t syclfft_hip_plan_scratch_bytes(const void *plan)
         return 0;
     }
     const auto *p = static_cast<const sfh::plan *>(plan);
+    if (p->real && p->core) {
+        const uint64_t own = p->real == 2 ? p->total / (2 * p->real_m) * (p->real_m + 1) * p->esize : 0;
+        return own + syclfft_hip_plan_scratch_bytes(p->half) + syclfft_hip_plan_scratch_bytes(p->core);
+    }
     if (p->real) {
         return p->cfg.batch_count * p->real_m * p->esize + syclfft_hip_plan_scratch_bytes(p->half);
     }
@@ -3225,6 +3312,9 @@ extern "C" uint32_t syclfft_hip_plan_launch_count(const void *plan)
         return 0;
     }
     const auto *p = static_cast<const sfh::plan *>(plan);
+    if (p->real && p->core) {
+        return syclfft_hip_plan_launch_count(p->half) + syclfft_hip_plan_launch_count(p->core);
+    }
     if (p->real) {
         return 1u + syclfft_hip_plan_launch_count(p->half);
     }

@@ -22,89 +22,124 @@
 extern "C" void oracle_fftn_ld(int rank, const uint64_t *lengths, uint64_t batch, const double *in,
                                double *out, int sign, int norm);
 
-// Real plans ('R' float, 'Q' double), N even: direction -1 = R2C (N real -> N/2 + 1 complex),
-// +1 = C2R (a Hermitian half spectrum -> N real); checked against the complex FP64 oracle.
+// Real plans ('R' float, 'Q' double), R:n0[xn1[xn2]]:batch:dir:norm:0 with the last length N
+// even: direction -1 = R2C (real [batch][..][N] -> complex [batch][..][N/2 + 1]), +1 = C2R
+// (a Hermitian half spectrum -> real); checked against the complex FP64 oracle.
 static bool run_real(const std::string &spec)
 {
-    char prec = 0;
-    unsigned long long n = 0, batch = 1;
+    char prec = 0, dims[128] = {0};
+    unsigned long long batch = 1;
     int dir = -1, norm = 0, inplace = 0;
-    if (std::sscanf(spec.c_str(), "%c:%llu:%llu:%d:%d:%d", &prec, &n, &batch, &dir, &norm, &inplace) != 6) {
+    if (std::sscanf(spec.c_str(), "%c:%127[0-9x]:%llu:%d:%d:%d", &prec, dims, &batch, &dir, &norm, &inplace) != 6) {
         std::printf("FAIL %s bad case spec\n", spec.c_str());
         return false;
     }
     const bool dbl = prec == 'Q';
-    const uint64_t M = n / 2;
     syclfft_hip_plan_config_v1 cfg{};
     cfg.abi_version = SYCLFFT_DEVICE_PROVIDER_ABI_VERSION;
     cfg.scalar = dbl ? SYCLFFT_HIP_SCALAR_DOUBLE : SYCLFFT_HIP_SCALAR_FLOAT;
     cfg.direction = dir;
-    cfg.rank = 1;
-    cfg.lengths[0] = n;
+    uint64_t lens[3] = {1, 1, 1}, T = 1;
+    {
+        std::string d(dims);
+        size_t pos = 0;
+        for (;;) {
+            const size_t x = d.find('x', pos);
+            lens[cfg.rank] = std::strtoull(d.substr(pos, x == std::string::npos ? std::string::npos : x - pos).c_str(),
+                                           nullptr, 10);
+            cfg.lengths[cfg.rank] = lens[cfg.rank];
+            T *= lens[cfg.rank];
+            ++cfg.rank;
+            if (x == std::string::npos || cfg.rank == 3) {
+                break;
+            }
+            pos = x + 1;
+        }
+    }
     cfg.batch_count = batch;
     cfg.normalization = static_cast<uint32_t>(norm);
     cfg.device = -1;
+    const uint64_t n = lens[cfg.rank - 1], hb = n / 2 + 1, rows = T / n;
     char err[1024];
     void *plan = nullptr;
     if (syclfft_hip_real_plan_create(&cfg, &plan, err, sizeof(err)) != 0) {
         std::printf("FAIL %s create: %s\n", spec.c_str(), err);
         return false;
     }
-    std::mt19937_64 rng(20261100 + n + batch);
+    std::mt19937_64 rng(20261100 + T + batch);
     std::uniform_real_distribution<double> u(-1.0, 1.0);
-    // a real signal r (batch x n) and its full spectrum R (FP64 oracle, no normalization)
-    std::vector<double> rc(2 * n * batch, 0.0), spec_full(2 * n * batch);
-    for (uint64_t i = 0; i < n * batch; ++i) {
+    // a real signal (batch x T) and its full spectrum (FP64 oracle, no normalization)
+    std::vector<double> rc(2 * T * batch, 0.0), spec_full(2 * T * batch);
+    for (uint64_t i = 0; i < T * batch; ++i) {
         const double v = u(rng);
         rc[2 * i] = dbl ? v : static_cast<double>(static_cast<float>(v));
     }
-    const uint64_t lens[1] = {n};
-    oracle_fftn_ld(1, lens, batch, rc.data(), spec_full.data(), -1, 0);
+    oracle_fftn_ld(static_cast<int>(cfg.rank), lens, batch, rc.data(), spec_full.data(), -1, 0);
     const size_t rs = dbl ? 8 : 4; // bytes per real value
-    const uint64_t in_vals = dir == -1 ? n * batch : 2 * (M + 1) * batch;
-    const uint64_t out_vals = dir == -1 ? 2 * (M + 1) * batch : n * batch;
+    const uint64_t half_vals = 2 * rows * hb * batch;
+    const uint64_t in_vals = dir == -1 ? T * batch : half_vals;
+    const uint64_t out_vals = dir == -1 ? half_vals : T * batch;
     void *din = nullptr, *dout = nullptr;
     hipMalloc(&din, in_vals * rs);
     hipMalloc(&dout, out_vals * rs);
     std::vector<double> in(in_vals), expect(out_vals);
+    // flat index of the element mirrored through every axis (the Hermitian partner)
+    const auto mirror = [&](uint64_t e) {
+        uint64_t m = 0, rest = e, mul = 1;
+        std::vector<uint64_t> idx(cfg.rank);
+        for (int d = static_cast<int>(cfg.rank) - 1; d >= 0; --d) {
+            idx[d] = rest % lens[d];
+            rest /= lens[d];
+        }
+        for (int d = static_cast<int>(cfg.rank) - 1; d >= 0; --d) {
+            m += ((lens[d] - idx[d]) % lens[d]) * mul;
+            mul *= lens[d];
+        }
+        return m;
+    };
     if (dir == -1) {
-        for (uint64_t i = 0; i < n * batch; ++i) {
+        for (uint64_t i = 0; i < T * batch; ++i) {
             in[i] = rc[2 * i];
         }
-        std::vector<double> ref(2 * n * batch);
-        oracle_fftn_ld(1, lens, batch, rc.data(), ref.data(), -1, norm);
+        std::vector<double> ref(2 * T * batch);
+        oracle_fftn_ld(static_cast<int>(cfg.rank), lens, batch, rc.data(), ref.data(), -1, norm);
         for (uint64_t b = 0; b < batch; ++b) {
-            for (uint64_t k = 0; k <= M; ++k) {
-                expect[2 * (b * (M + 1) + k)] = ref[2 * (b * n + k)];
-                expect[2 * (b * (M + 1) + k) + 1] = ref[2 * (b * n + k) + 1];
+            for (uint64_t r = 0; r < rows; ++r) {
+                for (uint64_t k = 0; k < hb; ++k) {
+                    const uint64_t o = (b * rows + r) * hb + k, f = b * T + r * n + k;
+                    expect[2 * o] = ref[2 * f];
+                    expect[2 * o + 1] = ref[2 * f + 1];
+                }
             }
         }
     } else {
-        // the input: bins 0..M of the spectrum (rounded to the plan's type); the expected
-        // output: the oracle's inverse of the full Hermitian spectrum those bins define
-        std::vector<double> full(2 * n * batch), ref(2 * n * batch);
+        // the input: the half spectrum, made exactly Hermitian (S[e] = conj S[mirror e]) and
+        // rounded to the plan's type; the expected output: the oracle's inverse of the full
+        // spectrum that half defines
+        std::vector<double> full(2 * T * batch), ref(2 * T * batch);
         for (uint64_t b = 0; b < batch; ++b) {
-            for (uint64_t k = 0; k <= M; ++k) {
-                double re = spec_full[2 * (b * n + k)], im = spec_full[2 * (b * n + k) + 1];
+            const double *S = spec_full.data() + 2 * b * T;
+            double *F = full.data() + 2 * b * T;
+            for (uint64_t e = 0; e < T; ++e) {
+                const uint64_t m = mirror(e);
+                double re = 0.5 * (S[2 * e] + S[2 * m]), im = 0.5 * (S[2 * e + 1] - S[2 * m + 1]);
                 if (!dbl) {
                     re = static_cast<float>(re);
                     im = static_cast<float>(im);
                 }
-                if (k == 0 || k == M) {
-                    im = 0.0; // a Hermitian spectrum is real at DC and Nyquist
-                }
-                in[2 * (b * (M + 1) + k)] = re;
-                in[2 * (b * (M + 1) + k) + 1] = im;
-                full[2 * (b * n + k)] = re;
-                full[2 * (b * n + k) + 1] = im;
-                if (k && k < M) {
-                    full[2 * (b * n + n - k)] = re;
-                    full[2 * (b * n + n - k) + 1] = -im;
+                F[2 * e] = re;
+                F[2 * e + 1] = im;
+            }
+            for (uint64_t r = 0; r < rows; ++r) {
+                for (uint64_t k = 0; k < hb; ++k) {
+                    const uint64_t o = (b * rows + r) * hb + k, f = r * n + k;
+                    in[2 * o] = F[2 * f];
+                    in[2 * o + 1] = F[2 * f + 1];
                 }
             }
         }
-        oracle_fftn_ld(1, lens, batch, full.data(), ref.data(), 1, norm);
-        for (uint64_t i = 0; i < n * batch; ++i) {
+        oracle_fftn_ld(static_cast<int>(cfg.rank), lens, batch, full.data(), ref.data(), 1, norm);
+        for (uint64_t i = 0; i < T * batch; ++i) {
             expect[i] = ref[2 * i];
         }
     }
@@ -128,8 +163,8 @@ static bool run_real(const std::string &spec)
     }
     const double rel = std::sqrt(num / (den > 0 ? den : 1));
     const double u_ = dbl ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
-    const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(n))) * u_;
-    char desc[2048];
+    const double tol = 4.0 * std::max(1.0, std::log2(static_cast<double>(T))) * u_;
+    char desc[4096];
     syclfft_hip_plan_describe(plan, desc, sizeof(desc));
     for (char *c = desc; *c; ++c) {
         if (*c == '\n') {

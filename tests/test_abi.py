@@ -81,13 +81,13 @@ def test_invalid_configs_rejected(mutate, status, text):
 
 @pytest.mark.parametrize("mutate,status,text", [
     (lambda c: c.lengths.__setitem__(0, 7), 1, "even"),
-    (lambda c: (setattr(c, "rank", 2), c.lengths.__setitem__(1, 4)), 1, "1-D"),
+    (lambda c: (setattr(c, "rank", 2), c.lengths.__setitem__(1, 5)), 1, "even last length"),
     (lambda c: setattr(c, "in_place", 1), 1, "out of place"),
     (lambda c: setattr(c, "precision", 2), 4, "VALU"),
     (lambda c: setattr(c, "batch_count", 0), 1, "batch"),
 ])
 def test_invalid_real_configs_rejected(mutate, status, text):
-    """Real (R2C/C2R) plans: 1-D, even length, out of place, VALU precision; checked before
+    """Real (R2C/C2R) plans: even last length, out of place, VALU precision; checked before
     any device work."""
     lib = sh.load()
     cfg = sh.make_config([8], 1)

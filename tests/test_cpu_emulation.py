@@ -156,9 +156,13 @@ def test_emulated_work_queue_execution():
 def test_emulated_real_plans():
     """R2C (N real -> N/2 + 1 complex) and C2R plans ('R' float, 'Q' double) against the
     complex FP64 oracle: the half-length complex plan over the real data viewed as complex,
-    single tile, Bluestein (N/2 prime) and four-step half plans, every normalization."""
+    single tile, Bluestein (N/2 prime) and four-step half plans, every normalization; and
+    2-D / 3-D real plans (C2R inputs made exactly Hermitian)."""
     _run(["R:2:3:-1:0:0", "R:16:4:-1:1:0", "R:4096:2:-1:0:0", "Q:2520:2:-1:3:0", "R:30:3:1:0:0",
-          "R:4096:2:1:2:0", "Q:1000:2:1:3:0", "R:20014:1:-1:0:0", "Q:2:2:1:0:0", "R:65536:1:1:1:0"])
+          "R:4096:2:1:2:0", "Q:1000:2:1:3:0", "R:20014:1:-1:0:0", "Q:2:2:1:0:0", "R:65536:1:1:1:0",
+          # multi-D: the last axis as 1-D real rows, the leading axes over the half spectrum
+          "R:8x6:2:-1:0:0", "Q:8x6:2:1:3:0", "R:12x10x8:1:-1:2:0", "Q:5x14:3:1:1:0", "R:32x64:2:1:0:0",
+          "Q:7x9x10:1:-1:3:0", "R:3x1000:2:1:2:0"])
 
 
 def test_emulated_layout_plans():

@@ -115,3 +115,84 @@ def test_real_plan_buffer_checks():
     assert e.value.code == "invalid_argument"
     assert p.launch_count() == 2
     p.close()
+
+
+# multi-D real plans: the last axis as 1-D real rows, then the leading axes of the half spectrum
+MD_CASES = [([8, 6], 2), ([32, 64], 3), ([12, 10, 8], 1), ([5, 14], 3), ([64, 2048], 2), ([256, 256], 2),
+            ([7, 9, 10], 2), ([1000, 30], 1), ([2, 2], 4), ([17, 4096], 1), ([4, 8, 10008], 1)]
+
+
+def hermitian(S, lengths):
+    """S symmetrized to exact Hermitian symmetry over the transform axes (all but the first)."""
+    axes = tuple(range(1, 1 + len(lengths)))
+    M = np.conj(np.roll(np.flip(S, axes), 1, axes))
+    return 0.5 * (S + M)
+
+
+@pytest.mark.parametrize("double", [False, True])
+@pytest.mark.parametrize("lengths,batch", MD_CASES)
+def test_real_nd_vs_oracle(lengths, batch, double):
+    """R2C of a real [batch][lengths] array against the first N/2 + 1 bins of the last axis of
+    the complex FP64 oracle; C2R of that half spectrum (made exactly Hermitian, rounded to the
+    plan's type) against the oracle's inverse of the full spectrum it defines."""
+    T = int(np.prod(lengths))
+    n = lengths[-1]
+    h = n // 2 + 1
+    norm = ["none", "orthogonal", "forward", "backward"][(T + batch) % 4]
+    say(f"case real nd lengths={lengths} batch={batch} norm={norm} double={double}")
+    g = np.random.default_rng(20261500 + T + batch)
+    rt, ct = (np.float64, np.complex128) if double else (np.float32, np.complex64)
+    x = g.uniform(-1, 1, batch * T).astype(rt)
+    shape = (batch, *lengths)
+    # R2C
+    fwd_norm = norm if norm != "backward" else "none"
+    p = sh.Plan(lengths, batch, sh.FORWARD, double=double, normalization=fwd_norm, real=True)
+    xd = torch.from_numpy(x).to("cuda")
+    yd = torch.empty(batch * T // n * h, dtype=torch.complex128 if double else torch.complex64, device="cuda")
+    p.execute(xd, yd)
+    torch.cuda.synchronize()
+    say(f"  r2c {p.describe().strip()}")
+    p.close()
+    ref = oracle_fftn(x.astype(np.complex128), lengths, batch, -1, fwd_norm).reshape(shape)[..., :h]
+    err = rel_l2(yd.cpu().numpy(), ref)
+    say(f"  r2c rel-L2 {err:.3e} (tol {tol(T, double):.1e})")
+    assert err <= tol(T, double)
+    # C2R
+    S = oracle_fftn(g.uniform(-1, 1, batch * T).astype(np.complex128), lengths, batch, -1, "none").reshape(shape)
+    F = hermitian(S, lengths)
+    F = F.astype(ct).astype(np.complex128)  # rounded: conjugate pairs stay exact conjugates
+    half = np.ascontiguousarray(F[..., :h]).astype(ct)
+    bwd_norm = norm if norm != "forward" else "none"
+    expect = oracle_fftn(F, lengths, batch, 1, bwd_norm).real
+    q = sh.Plan(lengths, batch, sh.BACKWARD, double=double, normalization=bwd_norm, real=True)
+    hd = torch.from_numpy(half.reshape(-1)).to("cuda")
+    zd = torch.empty(batch * T, dtype=torch.float64 if double else torch.float32, device="cuda")
+    q.execute(hd, zd)
+    torch.cuda.synchronize()
+    say(f"  c2r {q.describe().strip()}")
+    q.close()
+    err = rel_l2(zd.cpu().numpy(), expect)
+    say(f"  c2r rel-L2 {err:.3e} (tol {tol(T, double):.1e})")
+    assert err <= tol(T, double)
+
+
+@pytest.mark.parametrize("lengths,batch", [([64, 2048], 2), ([32, 32, 30], 3)])
+def test_real_nd_round_trip(lengths, batch):
+    """Multi-D R2C then C2R with orthogonal normalization returns the input (fp32)."""
+    say(f"case real nd round trip lengths={lengths} batch={batch}")
+    T = int(np.prod(lengths))
+    h = lengths[-1] // 2 + 1
+    x = np.random.default_rng(20261600 + T).uniform(-1, 1, batch * T).astype(np.float32)
+    fwd = sh.Plan(lengths, batch, sh.FORWARD, normalization="orthogonal", real=True)
+    bwd = sh.Plan(lengths, batch, sh.BACKWARD, normalization="orthogonal", real=True)
+    xd = torch.from_numpy(x).to("cuda")
+    sd = torch.empty(batch * T // lengths[-1] * h, dtype=torch.complex64, device="cuda")
+    zd = torch.empty_like(xd)
+    fwd.execute(xd, sd)
+    bwd.execute(sd, zd)
+    torch.cuda.synchronize()
+    fwd.close()
+    bwd.close()
+    err = rel_l2(zd.cpu().numpy(), x)
+    say(f"  rel-L2 {err:.3e}")
+    assert err <= 2 * tol(T, False)
