@@ -120,14 +120,16 @@ SYCLFFT_HIP_API void *syclfft_hip_plan_create(
 SYCLFFT_HIP_API int syclfft_hip_plan_create_ex(
     const syclfft_hip_plan_config_v1 *config, void **plan, char *err, size_t cap);
 
-/* Build a real 1-D plan of even length N = lengths[0] (rank 1, out of place, precision VALU):
- * direction -1 is R2C (batch_count x N real values in, batch_count x (N/2 + 1) complex
- * values out, the non-redundant half of the spectrum), +1 is C2R (the inverse: N/2 + 1
- * complex values of a Hermitian spectrum in, N real values out).  Batches are contiguous
- * with those distances; normalization scales by N as for complex plans.  The plan runs as
- * a complex plan of length N/2 over the real data viewed as N/2 complex values plus one
- * elementwise pass, and is executed, described and destroyed like any other plan.  No
- * reference counterpart: the reference plans C2C transforms only (README.md:18-20). */
+/* Build a real plan of rank 1..3 whose last length N = lengths[rank - 1] is even (out of
+ * place, precision VALU): direction -1 is R2C (batch_count x prod(lengths) real values in,
+ * C order, and batch_count x prod(lengths) / N x (N/2 + 1) complex values out, the
+ * non-redundant half of the spectrum along the last axis), +1 is C2R (the inverse, from a
+ * Hermitian half spectrum).  Batches are contiguous with those distances; normalization
+ * scales by prod(lengths) as for complex plans.  The last axis runs as a complex plan of
+ * length N/2 over the real rows viewed as N/2 complex values plus one elementwise pass;
+ * the leading axes run as strided complex axes over the half spectrum.  Executed, described
+ * and destroyed like any other plan.  No reference counterpart: the reference plans C2C
+ * transforms only (README.md:18-20). */
 SYCLFFT_HIP_API int syclfft_hip_real_plan_create(
     const syclfft_hip_plan_config_v1 *config, void **plan, char *err, size_t cap);
 
