@@ -132,3 +132,92 @@ def test_captured_execute_many_affine_and_rejects_tables():
                 else:
                     assert rel_l2(got[k].astype(np.complex128), want[k]) <= tol([n], False), (r, k)
         del g
+
+
+# layout plans: fused padded rows, interleaved batches (strided-axis plan), planar gather / scatter
+LAYOUT_CASES = [
+    ([4096], 4, (1, 4096 + 5, 1, 4096 + 5, 0, 0)),
+    ([1000], 8, (8, 1, 8, 1, 0, 0)),
+    ([64, 48], 3, (1, 64 * 48, 1, 64 * 48, 1, 1)),
+]
+
+
+@pytest.mark.parametrize("lengths,batch,lay", LAYOUT_CASES, ids=lambda c: str(c))
+def test_captured_layout_replays(lengths, batch, lay):
+    """A captured layout execution (fused, strided-axis, or gather / core / scatter) replays
+    bit-identically to an eager execution of the same plan."""
+    say(f"case graph layout lengths={lengths} batch={batch} layout={lay}")
+    T = int(np.prod(lengths))
+    istride, idist, ostride, odist, ip, op = lay
+    iext = (batch - 1) * idist + (T - 1) * istride + 1
+    oext = (batch - 1) * odist + (T - 1) * ostride + 1
+    with sh.Plan(lengths, batch, sh.FORWARD, layout=sh.Layout(*lay)) as p:
+        def side(ext, planar):
+            if planar:
+                return (torch.zeros(ext, device="cuda"), torch.zeros(ext, device="cuda"))
+            return torch.zeros(ext, dtype=torch.complex64, device="cuda")
+        xs, ys = side(iext, ip), side(oext, op)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            p.execute_layout(xs, ys)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            p.execute_layout(xs, ys)
+        flat = lambda v: torch.cat([v[0], v[1]]) if isinstance(v, tuple) else v  # noqa: E731
+        for r in range(REPLAYS):
+            gen = torch.Generator(device="cuda").manual_seed(8100 + r)
+            for t in (xs if isinstance(xs, tuple) else (torch.view_as_real(xs),)):
+                t.copy_(torch.rand(t.shape, generator=gen, device="cuda") * 2 - 1)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            got = flat(ys).cpu().numpy().copy()
+            ye = side(oext, op)
+            p.execute_layout(xs, ye)
+            torch.cuda.synchronize()
+            assert np.array_equal(got, flat(ye).cpu().numpy()), f"replay {r} differs from the eager execution"
+        say(f"{REPLAYS} replays bit-identical to eager [{p.describe().strip()}]")
+        del g
+
+
+@pytest.mark.parametrize("lengths,batch,direction", [([4096], 8, -1), ([64, 48], 3, -1), ([64, 48], 3, 1),
+                                                     ([16, 10, 12], 2, 1)])
+def test_captured_real_replays(lengths, batch, direction):
+    """Captured real plans (1-D, and multi-D with the leading axes over the half spectrum)
+    replay bit-identically to eager executions; R2C replays also meet the oracle bound."""
+    say(f"case graph real lengths={lengths} batch={batch} dir={direction}")
+    T = int(np.prod(lengths))
+    H = T // lengths[-1] * (lengths[-1] // 2 + 1)
+    with sh.Plan(lengths, batch, direction, real=True) as p:
+        if direction == -1:
+            xs = torch.zeros(T * batch, device="cuda")
+            ys = torch.zeros(H * batch, dtype=torch.complex64, device="cuda")
+        else:
+            xs = torch.zeros(H * batch, dtype=torch.complex64, device="cuda")
+            ys = torch.zeros(T * batch, device="cuda")
+        warm(p, xs, ys, False)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            p.execute(xs, ys)
+        for r in range(REPLAYS):
+            gen = torch.Generator(device="cuda").manual_seed(8200 + r)
+            src = xs if direction == -1 else torch.view_as_real(xs)
+            src.copy_(torch.rand(src.shape, generator=gen, device="cuda") * 2 - 1)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            got = ys.cpu().numpy().copy()
+            ye = torch.empty_like(ys)
+            p.execute(xs, ye)
+            torch.cuda.synchronize()
+            assert np.array_equal(got, ye.cpu().numpy()), f"replay {r} differs from the eager execution"
+            if direction == -1:
+                x = xs.cpu().numpy().astype(np.complex128)
+                want = oracle_fftn(x, lengths, batch, -1, "none").reshape(batch, T // lengths[-1], lengths[-1])
+                want = want[..., : lengths[-1] // 2 + 1]
+                assert rel_l2(got.astype(np.complex128), want) <= tol(lengths, False), r
+        say(f"{REPLAYS} replays bit-identical to eager [{p.describe().strip()}]")
+        del g
