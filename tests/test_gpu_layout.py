@@ -222,3 +222,46 @@ def test_layout_errors():
     finally:
         planar.close()
         inter.close()
+
+
+@pytest.mark.parametrize("lay,tag", [((1, 4096 + 64, 1, 4096 + 64, 0, 0), "padded rows, fused"),
+                                     ((8192, 1, 8192, 1, 0, 0), "interleaved batches"),
+                                     ((1, 4096, 1, 4096, 1, 1), "planar, gather / scatter")])
+def test_layout_round_trip_full_size(lay, tag):
+    """At the measured size (fp32 N=4096 batch 8192, 256 MiB): forward then backward with
+    orthogonal normalization returns the input, and the output layout's gaps are untouched."""
+    n, b = 4096, 8192
+    say(f"case layout round trip n={n} batch={b} {tag}")
+    L = sh.Layout(*lay)
+    ext = (b - 1) * lay[1] + (n - 1) * lay[0] + 1
+    fwd = sh.Plan([n], b, sh.FORWARD, normalization="orthogonal", layout=L)
+    bwd = sh.Plan([n], b, sh.BACKWARD, normalization="orthogonal", layout=L)
+    try:
+        gen = torch.Generator(device="cuda").manual_seed(20261700)
+        if lay[4]:
+            x = tuple(torch.rand(ext, generator=gen, device="cuda") * 2 - 1 for _ in range(2))
+            y = tuple(torch.full((ext,), SENTINEL, device="cuda") for _ in range(2))
+            z = tuple(torch.full((ext,), SENTINEL, device="cuda") for _ in range(2))
+        else:
+            x = torch.view_as_complex(torch.rand((ext, 2), generator=gen, device="cuda") * 2 - 1)
+            y = torch.full((ext,), SENTINEL + 1j * SENTINEL, dtype=torch.complex64, device="cuda")
+            z = torch.full((ext,), SENTINEL + 1j * SENTINEL, dtype=torch.complex64, device="cuda")
+        fwd.execute_layout(x, y)
+        bwd.execute_layout(y, z)
+        torch.cuda.synchronize()
+        off = (torch.arange(b, device="cuda")[:, None] * lay[1] + torch.arange(n, device="cuda")[None, :] * lay[0]).reshape(-1)
+        mask = torch.ones(ext, dtype=torch.bool, device="cuda")
+        mask[off] = False
+        parts = lambda v: v if isinstance(v, tuple) else (torch.view_as_real(v)[..., 0], torch.view_as_real(v)[..., 1])  # noqa: E731
+        num = den = 0.0
+        for xs, zs in zip(parts(x), parts(z)):
+            d = (zs[off].double() - xs[off].double())
+            num += float((d * d).sum())
+            den += float((xs[off].double() ** 2).sum())
+            assert bool((zs[mask] == SENTINEL).all()), "a gap of the output layout was written"
+        err = (num / den) ** 0.5
+        say(f"  round-trip rel-L2 {err:.3e}")
+        assert err <= 2 * 4 * 12 * U[False]
+    finally:
+        fwd.close()
+        bwd.close()
