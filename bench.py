@@ -583,14 +583,15 @@ def main():
                          z=torch.empty_like(x), fwd=fwd, bwd=bwd))
         log(f"{name}: fwd plan {fwd.describe().strip()}")
 
-    # Timed step: the six executions back to back on one stream, each forward launch
-    # bracketed by HIP events on that stream (every forward plan of this workload is one
-    # kernel launch), so per-kernel durations are those of the kernel running alone.
+    # Timed step: the six executions back to back on one stream, with nothing else on the
+    # stream between the two events that bracket the K steps.
     events = {s["name"]: [] for s in subs}
 
     def step(timed=False):
         for s in subs:
             if timed:
+                # per-launch events (their own packets add ~4 us per launch to the step, so
+                # this form runs in a separate loop, outside the headline timing)
                 a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 a.record(stream)
                 s["fwd"].execute(s["x"], s["y"])
@@ -609,7 +610,7 @@ def main():
     stop = torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(args.steps):
-        step(timed=True)
+        step()
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -618,6 +619,42 @@ def main():
     step_flops = sum(2 * flops(s["n"], s["b"]) for s in subs)
     step_bytes = sum(2 * 2 * s["n"] * s["b"] * (16 if s["dbl"] else 8) for s in subs)
     value = step_flops / (ms_per_step * 1e-3) / 1e9
+
+    # The same step with each forward launch bracketed by HIP events on the stream (every
+    # forward plan of this workload is one kernel launch), reported beside the metric.
+    ev_steps = min(args.steps, 50)
+    a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(ev_steps):
+        step(timed=True)
+    b_.record(stream)
+    torch.cuda.synchronize()
+    events_step_ms = a.elapsed_time(b_) / ev_steps
+
+    # The step captured once into a HIP graph (kernel nodes only) and replayed K times.
+    graph_ms = None
+    try:
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(stream)
+        with torch.cuda.stream(side):
+            step()
+        stream.wait_stream(side)
+        torch.cuda.synchronize()
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr):
+            step()
+        for _ in range(2):
+            gr.replay()
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            gr.replay()
+        b_.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = a.elapsed_time(b_) / args.steps
+        del gr
+    except Exception as exc:  # reported, not fatal: the eager step is the metric
+        log(f"graph step not measured: {exc}")
 
     # Per-kernel durations.  Events bracketing one launch inside the step also time the
     # launch's own queueing (a few microseconds), so each forward kernel is also timed over
@@ -792,6 +829,9 @@ def main():
                    "three_streams_gflops": round(step_flops / (overlap_ms * 1e-3) / 1e9, 2) if overlap_ms else None,
                    "fwd_launch_ms": {k: round(v, 4) for k, v in fwd_ms.items()},
                    "fwd_launch_in_step_ms": {k: round(v, 4) for k, v in fwd_in_step_ms.items()},
+                   "events_step_ms": round(events_step_ms, 4),
+                   "graph_ms_per_step": round(graph_ms, 4) if graph_ms else None,
+                   "graph_gflops": round(step_flops / (graph_ms * 1e-3) / 1e9, 2) if graph_ms else None,
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -16,6 +16,7 @@ SFH_DECLARE(f64_128_16)
 SFH_DECLARE(f64_256_16)
 SFH_DECLARE(f64_512_16)
 SFH_DECLARE(f64_512_20)
+SFH_DECLARE(f64_256_16g)
 #undef SFH_DECLARE
 
 namespace
@@ -25,6 +26,7 @@ struct variant_spec
     void (*fill)(const void **);
     int nt;
     int e;
+    int tw_global; // twin of the (nt, e) variant with its twiddle tables in global memory
 };
 
 const variant_spec k_f32[] = {
@@ -37,6 +39,9 @@ const variant_spec k_f64[] = {
     {sfh_kernels_f64_64_16, 64, 16},   {sfh_kernels_f64_128_16, 128, 16},
     {sfh_kernels_f64_256_16, 256, 16}, {sfh_kernels_f64_512_16, 512, 16},
     {sfh_kernels_f64_512_20, 512, 20},
+    // 4 waves per SIMD and no tables in LDS: f64 lengths 2049..2560 (40 KiB tiles) run four
+    // workgroups per CU instead of three; chosen per launch by resident-workgroup count
+    {sfh_kernels_f64_256_16g, 256, 16, 1},
 };
 
 constexpr int k_nf32 = sizeof(k_f32) / sizeof(k_f32[0]);
@@ -63,6 +68,7 @@ void init_tables()
         g_f32[i].nt = k_f32[i].nt;
         g_f32[i].e = k_f32[i].e;
         g_f32[i].maxr = k_f32[i].e <= 16 ? 16 : 9;
+        g_f32[i].tw_global = k_f32[i].tw_global;
         allow_lds(g_f32[i]);
     }
     for (int i = 0; i < k_nf64; ++i) {
@@ -70,6 +76,7 @@ void init_tables()
         g_f64[i].nt = k_f64[i].nt;
         g_f64[i].e = k_f64[i].e;
         g_f64[i].maxr = k_f64[i].e <= 16 ? 16 : 9;
+        g_f64[i].tw_global = k_f64[i].tw_global;
         allow_lds(g_f64[i]);
     }
 }

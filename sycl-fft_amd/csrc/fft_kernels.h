@@ -30,6 +30,7 @@ struct sfh_kernel_variant
     int nt;                        // threads per workgroup
     int e;                         // elements per thread; LDS capacity = nt * e complex values
     int maxr;                      // largest radix compiled into the variant (16, or 9 when e > 16)
+    int tw_global;                 // 1: twiddle tables read from global memory, none in LDS
 };
 
 extern "C" int sfh_device_cus();

@@ -360,7 +360,24 @@ static void set_last(plan &p, hipStream_t stream)
 
 static bool variant_allowed(const plan &p, int i)
 {
+    if (p.variants[i].tw_global) {
+        return false; // global-table twins replace their LDS-table variant after planning
+    }
     return p.cap_limit <= 0 || p.variants[i].nt * p.variants[i].e <= p.cap_limit;
+}
+
+// The global-table twin of variant i (same threads and elements per thread), or -1.
+static int global_table_twin(const plan &p, int i)
+{
+    if (i < 0 || p.variants[i].tw_global) {
+        return -1;
+    }
+    for (int j = 0; j < p.nvariants; ++j) {
+        if (p.variants[j].tw_global && p.variants[j].nt == p.variants[i].nt && p.variants[j].e == p.variants[i].e) {
+            return j;
+        }
+    }
+    return -1;
 }
 
 static int max_cap(const plan &p)
@@ -1639,6 +1656,40 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
                 l.grid = static_cast<uint32_t>(cap);
                 l.desc += " persistent grid=" + std::to_string(cap);
             }
+        }
+    }
+    // Global-table twins: a launch whose tile data alone would let more workgroups share a
+    // CU than tile + LDS twiddle tables do (by the runtime's occupancy count, which includes
+    // the twin's register budget) runs the twin that reads its tables from global memory,
+    // when its tiles outnumber the resident slots of the LDS-table form.  Plain (FFT-only)
+    // launches only.  SYCLFFT_HIP_TW_GLOBAL=1 takes the twin wherever one exists, =0 never.
+    {
+        const char *ge = std::getenv("SYCLFFT_HIP_TW_GLOBAL");
+        const int force = ge ? std::atoi(ge) : -1;
+        const int cus = sfh_device_cus();
+        for (auto &l : p->launches) {
+            const int twin = force == 0 ? -1 : global_table_twin(*p, l.variant);
+            if (twin < 0) {
+                continue;
+            }
+            const uint32_t data_bytes = l.args.lds_tab * static_cast<uint32_t>(p->esize);
+            if (force != 1) {
+                if (l.mode != SFH_MI_PLAIN) {
+                    continue; // the fused modes spill at the twin's register budget
+                }
+                const int r0 = sfh_resident_groups(&p->variants[l.variant], l.mode, l.args.lds_bytes);
+                const int r1 = sfh_resident_groups(&p->variants[twin], l.mode, data_bytes);
+                if (cus <= 0 || r0 <= 0 || r1 <= r0
+                    || l.args.ntiles <= static_cast<uint64_t>(r0) * static_cast<uint64_t>(cus)) {
+                    continue;
+                }
+                if (l.grid < l.args.ntiles) {
+                    continue; // persistent form: its grid was sized for the LDS-table variant
+                }
+            }
+            l.variant = twin;
+            l.args.lds_bytes = data_bytes;
+            l.desc += " tw=global lds=" + std::to_string(data_bytes);
         }
     }
     if (!p->host_tables.empty()) {

@@ -31,8 +31,21 @@
 #define SFH_DYNAMIC_LDS(name) unsigned char *name = emu_dynamic_lds()
 #endif
 
+// SFH_TW_GLOBAL = 1 builds a variant that reads the pass and four-step twiddle tables (a
+// few KiB, L1/L2-resident) from global memory instead of staging them in LDS behind the
+// tile, so a tile whose data alone fits a smaller LDS share gets more resident workgroups.
+#ifndef SFH_TW_GLOBAL
+#define SFH_TW_GLOBAL 0
+#endif
+
 namespace sfh
 {
+#if SFH_TW_GLOBAL
+// that variant's instantiations carry their own names, so the two builds of one
+// (T, threads, elements) never share a kernel symbol
+inline namespace twg
+{
+#endif
 
 template <typename T>
 struct cx
@@ -1399,8 +1412,47 @@ struct occupancy
     static constexpr int BYTES = CAP * int(sizeof(cx<T>)) + 1024;
     static constexpr int WG = 163840 / BYTES;
     static constexpr int RAW = WG * (NT / 64) / 4;
+#ifdef SFH_WAVES_EU
+    // set per variant where its tiles are sized below CAP (register budget of that many waves)
+    static constexpr int WAVES = SFH_WAVES_EU;
+#else
     static constexpr int WAVES = RAW < 1 ? 1 : (RAW > 8 ? 8 : RAW);
+#endif
 };
+
+// Where a tile's twiddle tables live: staged into LDS at a.lds_tab by `finish` (which ends
+// with the barrier that publishes them), or read in place from the plan's table block.
+template <typename T, int NT>
+__device__ __forceinline__ void bind_tables(const sfh_tile_args &a, cx<T> *lds, tile_ctx<T> &c)
+{
+#if SFH_TW_GLOBAL
+    (void)lds;
+    const cx<T> *const src = static_cast<const cx<T> *>(a.twtab);
+    c.tw = src;
+    c.t4 = src + a.tw_entries;
+#else
+    c.tw = lds + a.lds_tab;
+    c.t4 = lds + a.lds_tab + a.tw_entries;
+#endif
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void stage_tables(const sfh_tile_args &a, cx<T> *lds, uint32_t tid)
+{
+#if !SFH_TW_GLOBAL
+    cx<T> *const tab = lds + a.lds_tab;
+    const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
+    const uint32_t tab_count = a.tw_entries + a.t4_entries;
+    for (uint32_t e = tid; e < tab_count; e += NT) {
+        tab[e] = tab_src[e];
+    }
+#else
+    (void)a;
+    (void)lds;
+    (void)tid;
+#endif
+    __syncthreads();
+}
 
 // One tile g of a launch: stage the twiddle tables, load, run the fused program, store.
 template <typename T, int NT, int E, int MODE>
@@ -1410,20 +1462,11 @@ __device__ __forceinline__ void one_tile(const sfh_tile_args &a, cx<T> *lds, uin
     if constexpr (split_prec(MODE) != 0) {
         split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
     }
-    cx<T> *const tab = lds + a.lds_tab;
-    const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
-    const uint32_t tab_count = a.tw_entries + a.t4_entries;
-    c.tw = tab;
-    c.t4 = tab + a.tw_entries;
+    bind_tables<T, NT>(a, lds, c);
     // Twiddle tables (a few KiB) go to LDS behind the tile.  They are loaded right after the
     // tile's pass-0 loads have been issued, so the two HBM/L2 round trips overlap instead of
     // running one after the other.
-    const auto finish_tables = [=]() {
-        for (uint32_t e = tid; e < tab_count; e += NT) {
-            tab[e] = tab_src[e];
-        }
-        __syncthreads();
-    };
+    const auto finish_tables = [=]() { stage_tables<T, NT>(a, lds, tid); };
     set_tile<T, MODE>(a, c, g, tid);
     if (a.flags & SFH_LOAD_LDS) {
         finish_tables();
@@ -1464,18 +1507,9 @@ void tile_fft(const sfh_tile_args a_byval)
     if constexpr (split_prec(MODE) != 0) {
         split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
     }
-    cx<T> *const tab = lds + a.lds_tab;
-    const cx<T> *const tab_src = static_cast<const cx<T> *>(a.twtab);
-    const uint32_t tab_count = a.tw_entries + a.t4_entries;
-    c.tw = tab;
-    c.t4 = tab + a.tw_entries;
+    bind_tables<T, NT>(a, lds, c);
     const uint32_t f = a.flags;
-    const auto finish_tables = [=]() {
-        for (uint32_t e = tid; e < tab_count; e += NT) {
-            tab[e] = tab_src[e];
-        }
-        __syncthreads();
-    };
+    const auto finish_tables = [=]() { stage_tables<T, NT>(a, lds, tid); };
     if constexpr (PREF) {
         // Persistent loop over tiles g = blockIdx.x, + gridDim.x, ...  The inputs of the next
         // tile's pass 0 are loaded into registers right after the current tile's pass 0, so
@@ -1543,4 +1577,7 @@ void tile_fft_queue(const sfh_queue_args q)
     }
 }
 
+#if SFH_TW_GLOBAL
+} // namespace twg
+#endif
 } // namespace sfh

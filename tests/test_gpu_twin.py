@@ -1,0 +1,76 @@
+"""GPU tests of the f64 global-table twin variant: batched f64 lengths whose 40 KiB tiles fit
+four workgroups per CU only without twiddle tables in LDS (2049..2560 points) run the twin
+once their tiles outnumber the LDS-table variant's resident slots.  Results are checked
+against the FP64 oracle (relative L2 <= 2 log2(N) u) and against the same plan built with
+the twin switched off (SYCLFFT_HIP_TW_GLOBAL=0): the twin reads the same table values and
+runs the same arithmetic, so the outputs are bit-identical.
+"""
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import oracle_fftn, rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+import syclfft_hip as sh  # noqa: E402
+
+U64 = 2.0 ** -53
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(n, batch, direction, norm, x, twin_env):
+    old = os.environ.get("SYCLFFT_HIP_TW_GLOBAL")
+    if twin_env is None:
+        os.environ.pop("SYCLFFT_HIP_TW_GLOBAL", None)
+    else:
+        os.environ["SYCLFFT_HIP_TW_GLOBAL"] = twin_env
+    try:
+        p = sh.Plan([n], batch, direction, double=True, normalization=norm)
+    finally:
+        if old is None:
+            os.environ.pop("SYCLFFT_HIP_TW_GLOBAL", None)
+        else:
+            os.environ["SYCLFFT_HIP_TW_GLOBAL"] = old
+    desc = p.describe().strip()
+    xd = torch.from_numpy(x).to("cuda")
+    yd = torch.empty_like(xd)
+    p.execute(xd, yd)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    p.close()
+    return y, desc
+
+
+@pytest.mark.parametrize("n,batch,direction,norm", [
+    (2520, 1024, -1, "orthogonal"),
+    (2187, 1024, 1, "none"),
+    (2400, 2048, -1, "forward"),
+    (2560, 1024, 1, "backward"),
+])
+def test_twin_matches_oracle_and_lds_tables(n, batch, direction, norm):
+    say(f"case twin n={n} batch={batch} dir={direction} norm={norm}")
+    g = np.random.default_rng(20260100 + n)
+    x = g.uniform(-1, 1, n * batch) + 1j * g.uniform(-1, 1, n * batch)
+    y_auto, d_auto = run(n, batch, direction, norm, x, None)
+    y_off, d_off = run(n, batch, direction, norm, x, "0")
+    say(f"  auto: {d_auto}")
+    say(f"  off:  {d_off}")
+    assert "tw=global" not in d_off
+    lp = int(re.search(r"lp=(\d+)", d_off).group(1))  # LDS slots of the tile (pads included)
+    if lp * 16 <= 40960:
+        # tile data within 40 KiB: four per CU without tables (the LDS-table form holds three)
+        assert "tw=global" in d_auto
+    want = oracle_fftn(x, [n], batch, direction, norm)
+    e = rel_l2(y_auto, want)
+    say(f"  rel-L2 vs oracle {e:.3e}")
+    assert e <= 2 * np.log2(n) * U64
+    assert np.array_equal(y_auto, y_off), "twin and LDS-table variant differ"
