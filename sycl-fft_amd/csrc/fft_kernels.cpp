@@ -17,6 +17,7 @@ SFH_DECLARE(f64_256_16)
 SFH_DECLARE(f64_512_16)
 SFH_DECLARE(f64_512_20)
 SFH_DECLARE(f64_256_16g)
+SFH_DECLARE(f64_512_16g)
 #undef SFH_DECLARE
 
 namespace
@@ -39,9 +40,11 @@ const variant_spec k_f64[] = {
     {sfh_kernels_f64_64_16, 64, 16},   {sfh_kernels_f64_128_16, 128, 16},
     {sfh_kernels_f64_256_16, 256, 16}, {sfh_kernels_f64_512_16, 512, 16},
     {sfh_kernels_f64_512_20, 512, 20},
-    // 4 waves per SIMD and no tables in LDS: f64 lengths 2049..2560 (40 KiB tiles) run four
-    // workgroups per CU instead of three; chosen per launch by resident-workgroup count
+    // 4 waves per SIMD and no tables in LDS: f64 tiles of 2049..2560 points (40 KiB) run four
+    // workgroups per CU instead of three, and of 4097..5120 points (80 KiB) two instead of
+    // one; chosen per launch by resident-workgroup count
     {sfh_kernels_f64_256_16g, 256, 16, 1},
+    {sfh_kernels_f64_512_16g, 512, 16, 1},
 };
 
 constexpr int k_nf32 = sizeof(k_f32) / sizeof(k_f32[0]);

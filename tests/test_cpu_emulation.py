@@ -154,13 +154,13 @@ def test_emulated_work_queue_execution():
 
 
 def test_emulated_global_table_twin():
-    """The f64 variant twin that reads its twiddle tables from global memory (no tables in
-    LDS, four workgroups per CU for 40 KiB tiles), forced on every launch that has a twin:
-    single tiles of 2049..4096 points in every normalization and direction, in place, a
-    multi-D axis, and a work-queue launch of twin plans."""
+    """The f64 variant twins that read their twiddle tables from global memory (no tables in
+    LDS), forced on every launch that has a twin: single tiles of 2049..5120 points in every
+    normalization and direction, in place, a multi-D axis, and a work-queue launch of twin
+    plans."""
     env = {"SYCLFFT_HIP_TW_GLOBAL": "1"}
     _run(["d:2520:3:-1:3:0", "d:2520:2:1:0:1", "d:2187:2:-1:1:0", "d:4096:2:-1:0:0", "d:3000:2:1:2:0",
-          "d:48x2520:1:-1:0:0", "d:2560:3:1:3:0"], env)
+          "d:48x2520:1:-1:0:0", "d:2560:3:1:3:0", "d:4608:2:-1:0:0", "d:5040:2:1:3:0", "d:4097:1:-1:1:1"], env)
     _run(["d:2520:2:1:3:0", "d:3000:3:-1:0:0", "f:256:5:-1:0:0"], dict(env, EMU_QUEUE="1"))
 
 

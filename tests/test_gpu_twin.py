@@ -1,6 +1,7 @@
-"""GPU tests of the f64 global-table twin variant: batched f64 lengths whose 40 KiB tiles fit
-four workgroups per CU only without twiddle tables in LDS (2049..2560 points) run the twin
-once their tiles outnumber the LDS-table variant's resident slots.  Results are checked
+"""GPU tests of the f64 global-table twin variants: batched f64 lengths whose tiles fit more
+workgroups per CU only without twiddle tables in LDS and at 128 VGPRs (2049..2560 points:
+four instead of three; 4097..5120 points: two instead of one) run the twin once their
+tiles outnumber the LDS-table variant's resident slots.  Results are checked
 against the FP64 oracle (relative L2 <= 2 log2(N) u) and against the same plan built with
 the twin switched off (SYCLFFT_HIP_TW_GLOBAL=0): the twin reads the same table values and
 runs the same arithmetic, so the outputs are bit-identical.
@@ -55,6 +56,9 @@ def run(n, batch, direction, norm, x, twin_env):
     (2187, 1024, 1, "none"),
     (2400, 2048, -1, "forward"),
     (2560, 1024, 1, "backward"),
+    (4608, 1024, -1, "none"),
+    (5040, 512, 1, "orthogonal"),
+    (4374, 1024, -1, "forward"),
 ])
 def test_twin_matches_oracle_and_lds_tables(n, batch, direction, norm):
     say(f"case twin n={n} batch={batch} dir={direction} norm={norm}")
@@ -66,8 +70,10 @@ def test_twin_matches_oracle_and_lds_tables(n, batch, direction, norm):
     say(f"  off:  {d_off}")
     assert "tw=global" not in d_off
     lp = int(re.search(r"lp=(\d+)", d_off).group(1))  # LDS slots of the tile (pads included)
-    if lp * 16 <= 40960:
-        # tile data within 40 KiB: four per CU without tables (the LDS-table form holds three)
+    nt = int(re.search(r"nt=(\d+)", d_off).group(1))
+    if (nt == 256 and lp * 16 <= 40960) or (nt == 512 and lp * 16 <= 81920):
+        # 256 threads, data within 40 KiB: four per CU without tables (three with them);
+        # 512 threads, data within 80 KiB: two per CU (one at the LDS-table variant's registers)
         assert "tw=global" in d_auto
     want = oracle_fftn(x, [n], batch, direction, norm)
     e = rel_l2(y_auto, want)
