@@ -270,8 +270,8 @@ def serving_requests(nreq, seed):
 
 def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, streams=8):
     """The serving loop under three loads over the same request stream: all requests at
-    once (batched capacity), one at a time (isolated latency), and Poisson arrivals at 80%
-    of the batched request rate (latency under load).  Request buffers are device-resident:
+    once (batched capacity, median of three replays), one at a time (isolated latency), and
+    Poisson arrivals at 80% of the batched request rate (latency under load).  Request buffers are device-resident:
     inputs read from one seeded buffer, each request's output in its own region of one
     arena.  Arrivals are replayed from native threads (syclfft_hip_server_replay)."""
     reqs = serving_requests(nreq, seed)
@@ -293,7 +293,10 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
         return max(c["done_at"] for c in recs) - min(c["submit_at"] for c in recs)
 
     srv.replay(requests)  # warm-up: plans for every shape become resident
-    batched = srv.replay(requests)
+    # The all-at-once capacity depends on how the replay threads' submissions happen to
+    # merge into launches, so it is the median of three replays (spread reported).
+    runs = sorted((srv.replay(requests) for _ in range(3)), key=span)
+    batched = runs[1]
     span_b = span(batched)
     rate = len(reqs) / span_b
     isolated = srv.replay(requests, sequential=True)
@@ -316,6 +319,7 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
                   f"{streams} streams; arrivals replayed natively",
         "batched_gflops": round(flops_total / span_b / 1e9, 2),
         "batched_request_rate": round(rate, 1),
+        "batched_request_rate_runs": [round(len(reqs) / span(r), 1) for r in runs],
         "offered_request_rate": round(0.8 * rate, 1),
         "loaded_gflops": round(flops_total / span_l / 1e9, 2),
         "loaded_frac_of_batched": round(span_b / span_l, 4),
