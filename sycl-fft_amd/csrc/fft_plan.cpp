@@ -1683,6 +1683,13 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
                     || l.args.ntiles <= static_cast<uint64_t>(r0) * static_cast<uint64_t>(cus)) {
                     continue;
                 }
+                // Measured on f64 2049..2560 points: three -> four workgroups per CU wins
+                // while it removes a tail round (batch 1024-2048: 1.03-1.21x) and loses up to
+                // 8% over many rounds (batch 4096), where the twin's spill and table reads
+                // outweigh the residency; one -> two wins at every batch (1.22-1.42x).
+                if (r1 < 2 * r0 && l.args.ntiles > 2 * static_cast<uint64_t>(r1) * static_cast<uint64_t>(cus)) {
+                    continue;
+                }
                 if (l.grid < l.args.ntiles) {
                     continue; // persistent form: its grid was sized for the LDS-table variant
                 }
