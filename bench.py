@@ -271,7 +271,8 @@ def serving_requests(nreq, seed):
 def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, streams=8):
     """The serving loop under three loads over the same request stream: all requests at
     once (batched capacity, median of three replays), one at a time (isolated latency), and
-    Poisson arrivals at 80% of the batched request rate (latency under load).  Request buffers are device-resident:
+    Poisson arrivals at 80% of the batched request rate (latency under load, the median of
+    three arrival streams by p99).  Request buffers are device-resident:
     inputs read from one seeded buffer, each request's output in its own region of one
     arena.  Arrivals are replayed from native threads (syclfft_hip_server_replay)."""
     reqs = serving_requests(nreq, seed)
@@ -300,15 +301,23 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
     span_b = span(batched)
     rate = len(reqs) / span_b
     isolated = srv.replay(requests, sequential=True)
-    g = np.random.default_rng(seed + 1)
-    arrivals = np.cumsum(g.exponential(1.0 / (0.8 * rate), len(reqs)))
-    loaded = srv.replay(requests, arrivals=arrivals)
+    lat_i = np.array([c["latency_s"] for c in isolated]) * 1e3
+    # Poisson arrivals at 80% of the capacity, three arrival streams (seeds seed+1..+3): a
+    # loaded replay of 1500 requests lasts ~40 ms, so its p99 (the 15th-slowest request)
+    # varies between streams; the run with the median p99 is reported, with all three.
+    loads = []
+    for k in range(3):
+        g = np.random.default_rng(seed + 1 + k)
+        arrivals = np.cumsum(g.exponential(1.0 / (0.8 * rate), len(reqs)))
+        recs = srv.replay(requests, arrivals=arrivals)
+        loads.append((float(np.percentile([c["latency_s"] for c in recs], 99)), recs))
+    loads.sort(key=lambda t: t[0])
+    loaded = loads[1][1]
     span_l = span(loaded)
     st = srv.stats()
     srv.close()
     del arena, src, src64, src32
     lat_l = np.array([c["latency_s"] for c in loaded]) * 1e3
-    lat_i = np.array([c["latency_s"] for c in isolated]) * 1e3
     slow = lat_l / np.maximum(lat_i, 1e-6)
     pct = lambda a, q: round(float(np.percentile(a, q)), 4)  # noqa: E731
     return {
@@ -328,6 +337,7 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
         "isolated_p50_ms": pct(lat_i, 50),
         "isolated_p99_ms": pct(lat_i, 99),
         "p99_over_isolated_p99": round(float(np.percentile(lat_l, 99) / np.percentile(lat_i, 99)), 3),
+        "p99_over_isolated_p99_runs": [round(t[0] * 1e3 / float(np.percentile(lat_i, 99)), 3) for t in loads],
         "per_request_slowdown_p50": pct(slow, 50),
         "per_request_slowdown_p99": pct(slow, 99),
         "mean_group_size_batched": round(float(np.mean([c["group_size"] for c in batched])), 2),
