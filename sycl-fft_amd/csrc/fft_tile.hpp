@@ -1420,8 +1420,9 @@ struct occupancy
 #endif
 };
 
-// Where a tile's twiddle tables live: staged into LDS at a.lds_tab by `finish` (which ends
-// with the barrier that publishes them), or read in place from the plan's table block.
+// Where a tile's twiddle tables live: staged into LDS at a.lds_tab by stage_tables (which
+// ends with the barrier that publishes them), or, in the SFH_TW_GLOBAL build, read in place
+// from the plan's table block (stage_tables then only keeps the barrier).
 template <typename T, int NT>
 __device__ __forceinline__ void bind_tables(const sfh_tile_args &a, cx<T> *lds, tile_ctx<T> &c)
 {
@@ -1463,9 +1464,9 @@ __device__ __forceinline__ void one_tile(const sfh_tile_args &a, cx<T> *lds, uin
         split_dft16_matrix<split_prec(MODE)>(c.amat, wave_lane());
     }
     bind_tables<T, NT>(a, lds, c);
-    // Twiddle tables (a few KiB) go to LDS behind the tile.  They are loaded right after the
-    // tile's pass-0 loads have been issued, so the two HBM/L2 round trips overlap instead of
-    // running one after the other.
+    // Twiddle tables (a few KiB) go to LDS behind the tile (except in the SFH_TW_GLOBAL
+    // build).  They are loaded right after the tile's pass-0 loads have been issued, so the
+    // two HBM/L2 round trips overlap instead of running one after the other.
     const auto finish_tables = [=]() { stage_tables<T, NT>(a, lds, tid); };
     set_tile<T, MODE>(a, c, g, tid);
     if (a.flags & SFH_LOAD_LDS) {
