@@ -732,30 +732,42 @@ def main():
     del xc, yc
 
     # HBM traffic per launch of the dominant kernel, from the committed rocprofv3 --pmc
-    # summary of this workload (lines "<sub-workload> ... traffic_bytes=<B>"), if present
-    traffic = None
+    # summary of this workload (lines "<sub-workload> ... traffic_bytes=<B> plan=<describe>").
+    # A row counts only while its recorded plan is the plan this run built: after a plan
+    # change the profiled number no longer describes the timed kernel, so it is reported
+    # as stale (null) instead.
+    traffic, traffic_note = None, "no profile row"
+    dom_plan = dom["fwd"].describe().strip().replace("\n", " | ")
     tfile = os.path.join(ROOT, "profiles", "traffic_c2_roundtrip.txt")
     if os.path.exists(tfile):
         with open(tfile) as f:
             for row in f:
-                fields = row.split()
-                if fields and fields[0] == dom["name"]:
-                    for fld in fields[1:]:
-                        if fld.startswith("traffic_bytes="):
-                            traffic = int(fld.split("=", 1)[1])
+                if not row.startswith(dom["name"] + " "):
+                    continue
+                head, _, plan_txt = row.partition(" plan=")
+                tb = [int(fld.split("=", 1)[1]) for fld in head.split() if fld.startswith("traffic_bytes=")]
+                if tb and plan_txt.strip() == dom_plan:
+                    traffic, traffic_note = tb[0], "profiled plan matches"
+                elif tb:
+                    traffic_note = "stale: profiled plan differs from this run's plan"
 
-    # ---------------- accuracy of each sub-workload on sampled lines vs the FP64 oracle
+    # ---------------- accuracy of each sub-workload vs the FP64 oracle on the first, middle
+    # and last transform of the batch; over the bound c log2(N) u the run fails
     acc = {}
+    acc_ok = True
     for s in subs:
-        n, lines = s["n"], 4
-        x = s["x"][: n * lines].cpu().numpy()
-        y = s["y"][: n * lines].cpu().numpy()
-        z = s["z"][: n * lines].cpu().numpy()
-        ref = oracle_fft(lib, x, n, lines, -1, 3)
-        acc[s["name"]] = {
-            "fwd_rel_l2": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
-            "roundtrip_rel_l2": float(np.linalg.norm(z - x) / np.linalg.norm(x)),
-        }
+        n = s["n"]
+        rows = sorted({0, s["b"] // 2, s["b"] - 1})
+        pick = lambda t: np.concatenate([t[i * n:(i + 1) * n].cpu().numpy() for i in rows])
+        x, y, z = pick(s["x"]), pick(s["y"]), pick(s["z"])
+        ref = oracle_fft(lib, x, n, len(rows), -1, 3)
+        fe = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+        re = float(np.linalg.norm(z - x) / np.linalg.norm(x))
+        c = 4.0 if s["n"] == 10007 else 2.0
+        bound = c * np.log2(n) * (2.0 ** -53 if s["dbl"] else 2.0 ** -24)
+        ok = bool(fe <= bound and re <= 2 * bound)
+        acc_ok = acc_ok and ok
+        acc[s["name"]] = {"fwd_rel_l2": fe, "roundtrip_rel_l2": re, "bound": bound, "lines": rows, "ok": ok}
 
     # ---------------- fp32 batched HBM sweep (512 MiB per buffer)
     sweep = {}
@@ -851,6 +863,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"tile_fft {dom['name']} forward (1 launch)",
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": round(dom_ms, 4),
+                     "traffic_source": traffic_note, "plan": dom_plan,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                      "scaled by the copy-kernel factor)"},
         "cpu_baseline": cpu,
@@ -866,6 +879,9 @@ def main():
         "serving": serving,
     }
     print(json.dumps(line))
+    if not acc_ok:
+        log(f"accuracy check failed: {acc}")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

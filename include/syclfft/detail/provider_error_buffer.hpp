@@ -1,21 +1,17 @@
-// Error text across the provider ABIs: truncated to capacity-1 bytes and NUL-terminated
-// (reference: include/syclfft/detail/provider_error_buffer.hpp:10-18).
+// Error text crossing a provider ABI: the caller passes a buffer of `capacity` bytes; the
+// provider leaves a NUL-terminated message in it, cut to capacity - 1 characters (the
+// convention of the reference's provider modules, include/syclfft/detail/provider_error_buffer.hpp).
 #pragma once
 
-#include <algorithm>
 #include <cstddef>
-#include <cstring>
 #include <string>
 
 namespace syclfft::detail
 {
     inline void set_error_buffer(char *output, std::size_t capacity, const std::string &message)
     {
-        if (!output || capacity == 0) {
-            return;
+        if (output != nullptr && capacity > 0) {
+            output[message.copy(output, capacity - 1)] = '\0';
         }
-        const auto count = std::min(capacity - 1, message.size());
-        std::memcpy(output, message.data(), count);
-        output[count] = '\0';
     }
 } // namespace syclfft::detail

@@ -1,14 +1,24 @@
-// Core: validation helpers and provider-module loading.
-// Behaviour follows the reference (src/core/provider_loader.cpp:112-232): element-count
-// checks, normalization scale, provider names, search order SYCLFFT_PLUGIN_PATH then
-// <dir of this library>/syclfft/providers, dlopen(RTLD_NOW | RTLD_LOCAL), ABI check.
+// Core library: shape validation, normalization factors, provider names, and the loading
+// of provider modules.
+//
+// Behaviour contract of the reference's core (src/core/provider_loader.cpp:112-232), which
+// this file implements with its own structure:
+//   * shapes: rank 1..3, every length and the batch count non-zero, the element count
+//     representable in size_t, else invalid_argument;
+//   * normalization: none -> 1; forward -> 1/N on forward plans; backward -> 1/N on backward
+//     plans; orthogonal -> 1/sqrt(N) on both (N = product of the lengths);
+//   * modules are `<name>.so` files, looked up in every SYCLFFT_PLUGIN_PATH entry
+//     (':'-separated, empty entries skipped) and then in `syclfft/providers` next to this
+//     library, never in the working directory; each candidate is opened with
+//     dlopen(RTLD_NOW | RTLD_LOCAL) and must export the entry point;
+//   * no usable candidate -> provider_unavailable (the message lists every attempt); a host
+//     module whose table is missing or has another ABI version -> plugin_error.
 #include <dlfcn.h>
 
 #include <cmath>
 #include <cstdlib>
-#include <filesystem>
 #include <limits>
-#include <sstream>
+#include <string_view>
 #include <syclfft/detail/provider_loader.hpp>
 #include <syclfft/detail/trace.hpp>
 
@@ -16,138 +26,133 @@ namespace syclfft::detail
 {
     namespace
     {
-        std::vector<std::string> split_paths(const char *value)
+        // Directory holding this shared library ("" if the dynamic linker cannot say).
+        std::string own_directory()
         {
-            std::vector<std::string> out;
-            if (!value) {
-                return out;
-            }
-            std::stringstream in(value);
-            std::string item;
-            while (std::getline(in, item, ':')) {
-                if (!item.empty()) {
-                    out.push_back(item);
-                }
-            }
-            return out;
-        }
-
-        std::filesystem::path this_library_dir()
-        {
-            Dl_info info{};
-            if (dladdr(reinterpret_cast<void *>(&this_library_dir), &info) == 0 || !info.dli_fname) {
+            static const int anchor = 0;
+            Dl_info where{};
+            if (dladdr(&anchor, &where) == 0 || where.dli_fname == nullptr) {
                 return {};
             }
-            return std::filesystem::path(info.dli_fname).parent_path();
+            const std::string_view file(where.dli_fname);
+            const auto slash = file.rfind('/');
+            return slash == std::string_view::npos ? std::string(".") : std::string(file.substr(0, slash));
+        }
+
+        // An opened module; closing is the shared owner's job.
+        std::shared_ptr<void> open_module(const std::string &path, std::string &why)
+        {
+            void *raw = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+            if (raw == nullptr) {
+                const char *msg = dlerror();
+                why = msg ? msg : "dlopen failed";
+                return {};
+            }
+            return std::shared_ptr<void>(raw, [](void *h) { dlclose(h); });
         }
     } // namespace
 
     std::size_t checked_element_count(const std::vector<std::size_t> &lengths, std::size_t batch_count)
     {
-        if (lengths.empty() || lengths.size() > 3) {
-            throw exception(error_code::invalid_argument, "FFT rank must be between 1 and 3");
+        const std::size_t rank = lengths.size();
+        if (rank < 1 || rank > 3) {
+            throw exception(error_code::invalid_argument, "a transform has 1, 2 or 3 dimensions");
         }
         if (batch_count == 0) {
-            throw exception(error_code::invalid_argument, "FFT batch count must be non-zero");
+            throw exception(error_code::invalid_argument, "the batch count must be at least 1");
         }
-        std::size_t count = batch_count;
-        for (const auto n : lengths) {
+        constexpr std::size_t top = std::numeric_limits<std::size_t>::max();
+        std::size_t total = batch_count;
+        for (std::size_t axis = 0; axis < rank; ++axis) {
+            const std::size_t n = lengths[axis];
             if (n == 0) {
-                throw exception(error_code::invalid_argument, "FFT dimensions must be non-zero");
+                throw exception(error_code::invalid_argument, "length " + std::to_string(axis) + " is zero");
             }
-            if (count > std::numeric_limits<std::size_t>::max() / n) {
-                throw exception(error_code::invalid_argument, "FFT element count overflows size_t");
+            if (total > top / n) {
+                throw exception(error_code::invalid_argument, "the element count does not fit in size_t");
             }
-            count *= n;
+            total *= n;
         }
-        return count;
+        return total;
     }
 
     double normalization_scale(normalization mode, direction dir, std::size_t transform_size)
     {
         const double n = static_cast<double>(transform_size);
-        switch (mode) {
-        case normalization::none:
-            return 1.0;
-        case normalization::forward:
-            return dir == direction::forward ? 1.0 / n : 1.0;
-        case normalization::backward:
-            return dir == direction::backward ? 1.0 / n : 1.0;
-        case normalization::orthogonal:
+        const bool divides = (mode == normalization::forward && dir == direction::forward)
+                             || (mode == normalization::backward && dir == direction::backward);
+        if (mode == normalization::orthogonal) {
             return 1.0 / std::sqrt(n);
         }
-        return 1.0;
+        return divides ? 1.0 / n : 1.0;
     }
 
     const char *provider_name(provider value) noexcept
     {
-        switch (value) {
-        case provider::automatic:
-            return "automatic";
-        case provider::portable_sycl:
-            return "portable_sycl";
-        case provider::fftw:
-            return "fftw";
-        case provider::cufft:
-            return "cufft";
-        case provider::rocfft:
-            return "rocfft";
-        case provider::onemkl:
-            return "onemkl";
-        case provider::hip_native:
-            return "hip_native";
-        }
-        return "unknown";
+        static constexpr const char *names[] = {"automatic", "portable_sycl", "fftw",      "cufft",
+                                                "rocfft",    "onemkl",        "hip_native"};
+        const auto i = static_cast<std::size_t>(value);
+        return i < sizeof(names) / sizeof(names[0]) ? names[i] : "unknown";
     }
 
     std::vector<std::string> provider_search_paths()
     {
-        auto out = split_paths(std::getenv("SYCLFFT_PLUGIN_PATH"));
-        const auto dir = this_library_dir();
-        if (!dir.empty()) {
-            out.push_back((dir / "syclfft" / "providers").string());
+        std::vector<std::string> dirs;
+        if (const char *env = std::getenv("SYCLFFT_PLUGIN_PATH")) {
+            std::string_view rest(env);
+            while (!rest.empty()) {
+                const auto colon = rest.find(':');
+                const auto entry = rest.substr(0, colon);
+                if (!entry.empty()) {
+                    dirs.emplace_back(entry);
+                }
+                rest = colon == std::string_view::npos ? std::string_view{} : rest.substr(colon + 1);
+            }
         }
-        return out;
+        const std::string here = own_directory();
+        if (!here.empty()) {
+            dirs.push_back(here + "/syclfft/providers");
+        }
+        return dirs;
     }
 
     loaded_provider_symbol load_provider_symbol(const std::string &provider_file, const std::string &symbol_name)
     {
-        std::string errors;
-        for (const auto &dir : provider_search_paths()) {
-            const auto path = std::filesystem::path(dir) / (provider_file + ".so");
-            void *raw = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
-            if (!raw) {
-                const char *msg = dlerror();
-                errors += path.string() + ": " + (msg ? msg : "load failed") + "; ";
-                continue;
-            }
-            std::shared_ptr<void> module(raw, [](void *h) {
-                if (h) {
-                    dlclose(h);
+        std::string attempts;
+        for (const std::string &dir : provider_search_paths()) {
+            const std::string path = dir + "/" + provider_file + ".so";
+            std::string why;
+            std::shared_ptr<void> module = open_module(path, why);
+            if (module) {
+                if (void *entry = dlsym(module.get(), symbol_name.c_str())) {
+                    trace(("core: " + symbol_name + " found in " + path).c_str());
+                    return {entry, std::move(module)};
                 }
-            });
-            void *symbol = dlsym(raw, symbol_name.c_str());
-            if (!symbol) {
-                errors += path.string() + ": entry point '" + symbol_name + "' missing; ";
-                continue;
+                why = "no entry point " + symbol_name;
             }
-            trace("core: provider symbol resolved");
-            return {symbol, std::move(module)};
+            attempts += (attempts.empty() ? "" : "; ") + path + " (" + why + ")";
         }
-        throw exception(
-            error_code::provider_unavailable, "Unable to load provider '" + provider_file + "': " + errors);
+        if (attempts.empty()) {
+            attempts = "no search directory";
+        }
+        throw exception(error_code::provider_unavailable,
+                        "provider module " + provider_file + " is not usable: " + attempts);
     }
 
     loaded_host_provider load_host_provider(const std::string &provider_file)
     {
-        trace("core: resolving host provider symbol");
-        auto loaded = load_provider_symbol(provider_file, "syclfft_get_host_provider_v1");
-        const auto getter = reinterpret_cast<syclfft_get_host_provider_v1_fn>(loaded.symbol);
-        const auto *api = getter();
-        if (!api || api->abi_version != SYCLFFT_PROVIDER_ABI_VERSION) {
-            throw exception(error_code::plugin_error, "Provider '" + provider_file + "' has an incompatible ABI");
+        loaded_provider_symbol found = load_provider_symbol(provider_file, "syclfft_get_host_provider_v1");
+        const auto get_table = reinterpret_cast<syclfft_get_host_provider_v1_fn>(found.symbol);
+        const syclfft_host_provider_v1 *table = get_table();
+        if (table == nullptr) {
+            throw exception(error_code::plugin_error, "provider module " + provider_file + " returned no host table");
         }
-        return {api, std::move(loaded.module)};
+        if (table->abi_version != SYCLFFT_PROVIDER_ABI_VERSION) {
+            throw exception(error_code::plugin_error,
+                            "provider module " + provider_file + " has host ABI " + std::to_string(table->abi_version)
+                                + ", expected " + std::to_string(SYCLFFT_PROVIDER_ABI_VERSION));
+        }
+        return {table, std::move(found.module)};
     }
 
 } // namespace syclfft::detail

@@ -93,7 +93,7 @@ namespace
         std::vector<syclfft::complex<S>> dsrc(src.begin(), src.end());
         dbuf<syclfft::complex<S>> in(n), out(n), rep(n);
         in.write(dsrc);
-        auto fft = syclfft::plan_dft_1d<S>(q, n, dir, tu::options(placement, norm, syclfft::provider::portable_sycl));
+        auto fft = syclfft::plan_dft_1d<S>(q, n, dir, tu::opts().placed(placement).norm(norm).on(syclfft::provider::portable_sycl));
         const auto dep = syclfft::event::record(q);
         const bool inplace = placement == syclfft::placement::in_place;
         auto ev = inplace ? fft.execute(in.p, dep) : fft.execute(in.p, out.p, dep);
@@ -180,7 +180,7 @@ try {
         }
         in.write(src);
         auto f = syclfft::plan_dft_1d<float>(q, 13, direction::forward);
-        auto b = syclfft::plan_dft_1d<float>(q, 13, direction::backward, tu::options(placement::out_of_place, normalization::backward));
+        auto b = syclfft::plan_dft_1d<float>(q, 13, direction::backward, tu::opts().norm(normalization::backward));
         const auto done = f.execute(in.p, spec.p);
         b.execute(spec.p, back.p, done).wait_and_throw();
         const auto r = back.read();
@@ -230,7 +230,7 @@ try {
     }
     using syclfft::error_code;
     tu::expect_error(error_code::provider_unavailable, [&] {
-        (void)syclfft::plan_dft_1d<float>(q, 8, direction::forward, tu::options(placement::out_of_place, normalization::none, syclfft::provider::cufft));
+        (void)syclfft::plan_dft_1d<float>(q, 8, direction::forward, tu::opts().on(syclfft::provider::cufft));
     }, "cufft");
     tu::expect_error(error_code::invalid_argument, [&] { (void)syclfft::plan_dft_1d<float>(q, 8, static_cast<direction>(0)); }, "direction");
     tu::expect_error(error_code::invalid_pointer, [&] {
@@ -249,7 +249,7 @@ try {
         f.execute(small.p, out.p);
     }, "buffer shorter than the transform");
     tu::expect_error(error_code::invalid_argument, [&] {
-        auto f = syclfft::plan_dft_1d<float>(q, 8, direction::forward, tu::options(placement::in_place));
+        auto f = syclfft::plan_dft_1d<float>(q, 8, direction::forward, tu::opts().in_place());
         dbuf<syclfft::complex<float>> a(8), b(8);
         f.execute(a.p, b.p);
     }, "placement");

@@ -69,7 +69,7 @@ static void run_case(std::vector<std::size_t> shape, std::size_t batch, syclfft:
         in[i] = {static_cast<S>((i * 7) % 13) / S{7}, static_cast<S>((i * 5 + 3) % 11) / S{9}};
     }
     const auto expected = reference(in, shape, batch, dir, norm);
-    auto fft = syclfft::host::plan_many_dft<S>(shape, batch, dir, tu::options(placement, norm, syclfft::provider::fftw));
+    auto fft = syclfft::host::plan_many_dft<S>(shape, batch, dir, tu::opts().placed(placement).norm(norm).on(syclfft::provider::fftw));
     tu::check(fft.selected_provider() == syclfft::provider::fftw, "wrong host provider");
     tu::check(fft.batch_count() == batch, "wrong batch count");
     tu::check(std::equal(fft.shape().begin(), fft.shape().end(), shape.begin()), "wrong shape");
@@ -103,7 +103,7 @@ try {
         }
         auto f = syclfft::host::plan_dft_1d<double>(13, syclfft::direction::forward);
         auto b = syclfft::host::plan_dft_1d<double>(13, syclfft::direction::backward,
-                                                     tu::options(syclfft::placement::out_of_place, syclfft::normalization::backward));
+                                                     tu::opts().norm(syclfft::normalization::backward));
         f.execute(in.data(), spec.data());
         b.execute(spec.data(), back.data());
         compare(back, in, 2e-11, "round trip");
@@ -120,7 +120,7 @@ try {
         f.execute(nullptr, nullptr);
     }, "null pointers");
     tu::expect_error(error_code::invalid_argument, [] {
-        auto f = syclfft::host::plan_dft_1d<float>(4, syclfft::direction::forward, tu::options(syclfft::placement::in_place));
+        auto f = syclfft::host::plan_dft_1d<float>(4, syclfft::direction::forward, tu::opts().in_place());
         std::vector<std::complex<float>> a(4), b(4);
         f.execute(a.data(), b.data());
     }, "placement");

@@ -1,4 +1,5 @@
-// Check helpers of the C++ API tests (same contract as the reference's tests/test_utils.hpp:9-40).
+// Scaffolding of the C++ plan-API test programs: checks that count themselves, plan
+// options assembled by name, and an expectation on the error code a call raises.
 #pragma once
 
 #include <stdexcept>
@@ -7,36 +8,72 @@
 
 namespace tu
 {
-    inline void check(bool condition, const std::string &message)
+    inline int &checks_run()
     {
-        if (!condition) {
-            throw std::runtime_error(message);
+        static int n = 0;
+        return n;
+    }
+
+    inline void check(bool ok, const std::string &what)
+    {
+        ++checks_run();
+        if (!ok) {
+            throw std::runtime_error("check failed: " + what);
         }
     }
 
-    inline syclfft::plan_options options(
-        syclfft::placement placement = syclfft::placement::out_of_place,
-        syclfft::normalization normalization = syclfft::normalization::none,
-        syclfft::provider provider = syclfft::provider::automatic,
-        syclfft::planning_mode planning = syclfft::planning_mode::estimate)
+    // plan_options by name, e.g. tu::opts().in_place().norm(normalization::forward).on(provider::fftw)
+    class opts
     {
-        syclfft::plan_options o;
-        o.placement = placement;
-        o.normalization = normalization;
-        o.preferred_provider = provider;
-        o.planning = planning;
-        return o;
-    }
+    public:
+        opts &placed(syclfft::placement p)
+        {
+            o_.placement = p;
+            return *this;
+        }
+        opts &in_place()
+        {
+            return placed(syclfft::placement::in_place);
+        }
+        opts &norm(syclfft::normalization n)
+        {
+            o_.normalization = n;
+            return *this;
+        }
+        opts &on(syclfft::provider p)
+        {
+            o_.preferred_provider = p;
+            return *this;
+        }
+        opts &measured()
+        {
+            o_.planning = syclfft::planning_mode::measure;
+            return *this;
+        }
+        operator syclfft::plan_options() const
+        {
+            return o_;
+        }
 
+    private:
+        syclfft::plan_options o_{};
+    };
+
+    // f() must throw syclfft::exception carrying `code`; returns the exception's message.
     template <class F>
-    void expect_error(syclfft::error_code code, F &&f, const std::string &what)
+    std::string expect_error(syclfft::error_code code, F &&f, const std::string &what)
     {
+        std::string message;
+        bool raised = false;
         try {
             f();
         } catch (const syclfft::exception &ex) {
-            check(ex.code() == code, what + ": wrong error code (" + ex.what() + ")");
-            return;
+            raised = true;
+            message = ex.what();
+            check(ex.code() == code, what + ": error code " + std::to_string(static_cast<int>(ex.code())) + " ("
+                                         + message + "), expected " + std::to_string(static_cast<int>(code)));
         }
-        throw std::runtime_error(what + ": expected syclfft::exception");
+        check(raised, what + ": no syclfft::exception");
+        return message;
     }
 } // namespace tu

@@ -69,6 +69,22 @@ def test_reference_plugin_test_against_bad_module():
     run(os.path.join(REF, "ref_plugin_tests"), {"SYCLFFT_PLUGIN_PATH": os.path.join(REF, "bad")})
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "ours_host_tests")),
+                    reason="reference sources not present")
+def test_reference_host_tests_against_our_front_end():
+    """The reference's tests/host_tests.cpp compiled against this repository's host::plan
+    front-end and provider loader (include/, sycl-fft_amd/lib)."""
+    if not fftw_backend_available():
+        pytest.skip("no FFTW3-API library on this host")
+    assert "passed" in run(os.path.join(REF, "ours_host_tests"), {"SYCLFFT_PLUGIN_PATH": PROVIDERS})
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "ours_plugin_tests")),
+                    reason="reference sources not present")
+def test_reference_plugin_test_against_our_loader():
+    run(os.path.join(REF, "ours_plugin_tests"), {"SYCLFFT_PLUGIN_PATH": os.path.join(REF, "bad")})
+
+
 @pytest.mark.gpu
 def test_device_plan_api():
     assert "passed" in run(os.path.join(BIN, "device_tests"), timeout=120)

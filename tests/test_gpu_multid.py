@@ -48,14 +48,18 @@ def tones_input(shape, double, seed):
     return x.to(cdt).contiguous(), freqs, amps
 
 
+@pytest.mark.parametrize("measure", [False, True], ids=["estimate", "measure"])
 @pytest.mark.parametrize("shape,double", [((256, 256, 256), False), ((4096, 4096), True)],
                          ids=["256^3_f32", "4096^2_f64"])
-def test_multid_tone_peaks_roundtrip(shape, double):
+def test_multid_tone_peaks_roundtrip(shape, double, measure):
+    """measure=True builds the plans bench.py times (planning_mode::measure)."""
     n = int(np.prod(shape))
-    say(f"case multi-D {shape} double={double}")
+    say(f"case multi-D {shape} double={double} measure={measure}")
     x, freqs, amps = tones_input(shape, double, 20260003)
     x0 = x.clone()
-    with sh.Plan(list(shape), 1, sh.FORWARD, double=double, in_place=True, normalization="forward") as fwd:
+    with sh.Plan(list(shape), 1, sh.FORWARD, double=double, in_place=True, normalization="forward",
+                 measure=measure) as fwd:
+        say("plan " + fwd.describe().strip().replace("\n", " | "))
         fwd.execute(x)
     torch.cuda.synchronize()
     # peaks: the 8 tone bins carry the amplitudes, every other bin only noise
