@@ -1664,8 +1664,16 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     // when its tiles outnumber the resident slots of the LDS-table form.  Plain (FFT-only)
     // launches only.  SYCLFFT_HIP_TW_GLOBAL=1 takes the twin wherever one exists, =0 never.
     {
-        const char *ge = std::getenv("SYCLFFT_HIP_TW_GLOBAL");
-        const int force = ge ? std::atoi(ge) : -1;
+        int force = -1;
+        if (const char *ge = std::getenv("SYCLFFT_HIP_TW_GLOBAL")) {
+            if (std::strcmp(ge, "0") == 0) {
+                force = 0;
+            } else if (std::strcmp(ge, "1") == 0) {
+                force = 1;
+            } else {
+                throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "SYCLFFT_HIP_TW_GLOBAL is not 0 or 1");
+            }
+        }
         const int cus = sfh_device_cus();
         for (auto &l : p->launches) {
             const int twin = force == 0 ? -1 : global_table_twin(*p, l.variant);
@@ -1673,10 +1681,13 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
                 continue;
             }
             const uint32_t data_bytes = l.args.lds_tab * static_cast<uint32_t>(p->esize);
+            // plain one-tile-per-workgroup launches only, also when forced: the fused modes
+            // spill at the twin's register budget, and a persistent grid was sized for the
+            // LDS-table variant
+            if (l.mode != SFH_MI_PLAIN || l.grid < l.args.ntiles) {
+                continue;
+            }
             if (force != 1) {
-                if (l.mode != SFH_MI_PLAIN) {
-                    continue; // the fused modes spill at the twin's register budget
-                }
                 const int r0 = sfh_resident_groups(&p->variants[l.variant], l.mode, l.args.lds_bytes);
                 const int r1 = sfh_resident_groups(&p->variants[twin], l.mode, data_bytes);
                 if (cus <= 0 || r0 <= 0 || r1 <= r0
@@ -1689,9 +1700,6 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
                 // outweigh the residency; one -> two wins at every batch (1.22-1.42x).
                 if (r1 < 2 * r0 && l.args.ntiles > 2 * static_cast<uint64_t>(r1) * static_cast<uint64_t>(cus)) {
                     continue;
-                }
-                if (l.grid < l.args.ntiles) {
-                    continue; // persistent form: its grid was sized for the LDS-table variant
                 }
             }
             l.variant = twin;
@@ -2072,9 +2080,11 @@ static void execute_layout(
     plan &p, const void *in, const void *in_im, void *out, void *out_im, hipStream_t stream, void *const *deps,
     size_t ndeps, hipEvent_t done);
 
-// The launches of a complex plan over buffers already checked.
+// The launches of a complex plan over buffers already checked.  `scratch` is the plan's own
+// scratch, or caller-provided device memory of at least scratch_elems elements.
 static void execute_checked(
-    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done)
+    plan &p, const void *in, void *out, hipStream_t stream, void *const *deps, size_t ndeps, hipEvent_t done,
+    void *scratch)
 {
     std::lock_guard<std::mutex> lock(p.mu);
     device_guard guard(p.device);
@@ -2095,7 +2105,7 @@ static void execute_checked(
                 sfh_launch_copy_scale(in, out, p.total, p.dbl ? 1 : 0, p.scale, stream)),
             "copy_scale launch");
     }
-    void *bufs[3] = {const_cast<void *>(in), out, p.scratch};
+    void *bufs[3] = {const_cast<void *>(in), out, scratch};
     for (auto &l : p.launches) {
         sfh_tile_args a = l.args;
         a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
@@ -2130,7 +2140,7 @@ static void execute_plan(
         return;
     }
     check_buffers(p, in, out);
-    execute_checked(p, in, out, stream, deps, ndeps, done);
+    execute_checked(p, in, out, stream, deps, ndeps, done, p.scratch);
 }
 
 // Execution of `count` transforms of the plan's shape given by pointer arrays: transform k
@@ -2655,9 +2665,9 @@ static void execute_real(
     if (p.core) { // multi-D: the last axis as 1-D real rows plus the leading axes
         if (p.real == 1) {
             execute_plan(*p.half, in, out, stream, nullptr, 0, nullptr);
-            execute_checked(*p.core, out, out, stream, nullptr, 0, nullptr);
+            execute_checked(*p.core, out, out, stream, nullptr, 0, nullptr, p.core->scratch);
         } else {
-            execute_checked(*p.core, in, p.real_scratch, stream, nullptr, 0, nullptr);
+            execute_checked(*p.core, in, p.real_scratch, stream, nullptr, 0, nullptr, p.core->scratch);
             execute_plan(*p.half, p.real_scratch, out, stream, nullptr, 0, nullptr);
         }
     } else if (p.real == 1) {
@@ -2921,14 +2931,14 @@ static void execute_layout(
     }
     const int dbl = p.dbl ? 1 : 0;
     if (p.lay_fused) {
-        execute_checked(*p.core, in, out, stream, nullptr, 0, nullptr);
+        execute_checked(*p.core, in, out, stream, nullptr, 0, nullptr, p.core->scratch);
     } else {
         const int64_t t = static_cast<int64_t>(T);
         hip_check(static_cast<hipError_t>(sfh_launch_layout_copy(
                       in, in_im, static_cast<int64_t>(L.in_stride), static_cast<int64_t>(L.in_distance),
                       static_cast<int>(L.in_planar), p.lay_scratch, nullptr, 1, t, 0, T, B, dbl, stream)),
                   "layout gather launch");
-        execute_checked(*p.core, p.lay_scratch, p.lay_scratch, stream, nullptr, 0, nullptr);
+        execute_checked(*p.core, p.lay_scratch, p.lay_scratch, stream, nullptr, 0, nullptr, p.core->scratch);
         hip_check(static_cast<hipError_t>(sfh_launch_layout_copy(
                       p.lay_scratch, nullptr, 1, t, 0, out, out_im, static_cast<int64_t>(L.out_stride),
                       static_cast<int64_t>(L.out_distance), static_cast<int>(L.out_planar), T, B, dbl, stream)),
