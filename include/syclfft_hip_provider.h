@@ -192,6 +192,23 @@ SYCLFFT_HIP_API int syclfft_hip_plan_execute(
     void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
     void *done, char *err, size_t cap);
 
+/* As syclfft_hip_plan_execute for complex plans, but allocating nothing: launches that pass
+ * intermediates through scratch (four-step axes, the three-launch Bluestein form) use a
+ * file-scope device array of syclfft_hip_noalloc_scratch_limit() bytes (64 MiB) per device,
+ * reserved when the library is loaded on a device and held until the process ends, instead
+ * of the plan's own scratch.  A plan whose scratch_bytes exceed that limit, and real or
+ * layout plans, are rejected with SYCLFFT_HIP_STATUS_INVALID_ARGUMENT.  Every call on one
+ * device shares that device's array: calls whose executions overlap on one device race,
+ * so the caller orders them.  Results are bit-identical to syclfft_hip_plan_execute.
+ * No counterpart in the reference: for callers that cannot allocate device memory while
+ * they run (HIP graph capture, serving loops, harnesses that forbid hipMalloc). */
+SYCLFFT_HIP_API int syclfft_hip_plan_execute_noalloc(
+    void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps,
+    void *done, char *err, size_t cap);
+
+/* Bytes of the device array syclfft_hip_plan_execute_noalloc uses as scratch. */
+SYCLFFT_HIP_API uint64_t syclfft_hip_noalloc_scratch_limit(void);
+
 /* Enqueue `count` transforms of the plan's shape (1 <= count <= the plan's batch count)
  * given by host arrays of device pointers: transform k reads ins[k] and writes outs[k]
  * (ins[k] == outs[k] for in-place plans).  Same ordering, dependency and completion

@@ -3259,6 +3259,57 @@ extern "C" int syclfft_hip_plan_execute(
     }
 }
 
+// Execution of a complex plan with `scratch` (device memory of `scratch_bytes` bytes on the
+// plan's device) in place of the plan's own scratch; nothing is allocated.  Backs
+// syclfft_hip_plan_execute_noalloc (fft_noalloc.hip), which owns the scratch memory.
+extern "C" __attribute__((visibility("hidden"))) int sfh_plan_execute_with_scratch(
+    void *plan, const void *in, void *out, void *stream, void *const *deps, size_t ndeps, void *done,
+    void *scratch, uint64_t scratch_bytes, char *err, size_t cap)
+{
+    try {
+        sfh::set_err(err, cap, "");
+        if (!plan) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_STATE, "plan handle is null");
+        }
+        if (ndeps && !deps) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "dependency list is null");
+        }
+        sfh::plan &p = *static_cast<sfh::plan *>(plan);
+        if (p.real || p.layout) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                            "the no-allocation entry point runs complex plans only (not real or layout plans)");
+        }
+        const uint64_t need = p.scratch_elems * p.esize;
+        if (need > scratch_bytes) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT,
+                            "plan needs " + std::to_string(need) + " bytes of scratch; the no-allocation entry point holds "
+                                + std::to_string(scratch_bytes));
+        }
+        if (need && !scratch) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_POINTER, "no scratch memory on the plan's device");
+        }
+        sfh::check_buffers(p, in, out);
+        sfh::execute_checked(p, in, out, static_cast<hipStream_t>(stream), deps, ndeps, static_cast<hipEvent_t>(done),
+                             scratch);
+        return SYCLFFT_HIP_STATUS_OK;
+    } catch (const sfh_error &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return ex.status;
+    } catch (const std::exception &ex) {
+        sfh::set_err(err, cap, ex.what());
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    } catch (...) {
+        sfh::set_err(err, cap, "unknown error");
+        return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
+    }
+}
+
+// Device of a plan (-1 for a null handle).
+extern "C" __attribute__((visibility("hidden"))) int sfh_plan_device(const void *plan)
+{
+    return plan ? static_cast<const sfh::plan *>(plan)->device : -1;
+}
+
 extern "C" int syclfft_hip_plan_execute_many(
     void *plan, uint64_t count, const void *const *ins, void *const *outs, void *stream, void *const *deps,
     size_t ndeps, void *done, char *err, size_t cap)

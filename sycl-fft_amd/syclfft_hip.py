@@ -51,6 +51,8 @@ EXPORTED_SYMBOLS = (
     "syclfft_hip_plan_destroy",
     "syclfft_hip_plan_scratch_bytes",
     "syclfft_hip_plan_execute",
+    "syclfft_hip_plan_execute_noalloc",
+    "syclfft_hip_noalloc_scratch_limit",
     "syclfft_hip_plan_execute_many",
     "syclfft_hip_execute_queue",
     "syclfft_hip_plan_describe",
@@ -164,6 +166,10 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
         ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
         ctypes.c_size_t]
     lib.syclfft_hip_plan_execute.restype = ctypes.c_int
+    lib.syclfft_hip_plan_execute_noalloc.argtypes = lib.syclfft_hip_plan_execute.argtypes
+    lib.syclfft_hip_plan_execute_noalloc.restype = ctypes.c_int
+    lib.syclfft_hip_noalloc_scratch_limit.argtypes = []
+    lib.syclfft_hip_noalloc_scratch_limit.restype = ctypes.c_uint64
     lib.syclfft_hip_plan_execute_many.argtypes = [
         ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
         ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p,
