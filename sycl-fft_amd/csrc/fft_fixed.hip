@@ -1,0 +1,63 @@
+// Length-specialized tile kernels (fft_fixed.hpp) and their registry.  Each entry names
+// its precision, fusion mode, length, workgroup size and radix program; the planner
+// (fft_plan.cpp) replaces a generic launch of the same arguments by an entry whose
+// (precision, mode, length) match, and builds the entry's per-pass twiddle table from
+// its radix program.
+#include "fft_fixed.hpp"
+#include "fft_kernels.h"
+
+namespace
+{
+using sfh::fx::radices;
+
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, class R1, class R2>
+sfh_fixed_kernel entry()
+{
+    using prog = sfh::fx::program<R1, R2, MODE, PAD>;
+    sfh_fixed_kernel k{};
+    k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, R1, R2>);
+    k.is_double = sizeof(T) == 8;
+    k.mode_index = (MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED : SFH_MI_PLAIN;
+    k.nt = NT;
+    k.L = static_cast<uint32_t>(prog::L);
+    // LDS bytes per tile: one complex value per slot, or one real with the split exchange
+    k.lds_bytes = static_cast<uint32_t>(prog::slots() * (SPLIT ? sizeof(T) : 2 * sizeof(T)));
+    k.n1 = R1::n;
+    for (int i = 0; i < R1::n; ++i) {
+        k.r1[i] = R1::v[i];
+    }
+    k.n2 = (MODE & SFH_M_MID) ? R2::n : 0;
+    for (int i = 0; i < k.n2; ++i) {
+        k.r2[i] = R2::v[i];
+    }
+    return k;
+}
+
+constexpr int BLUE = SFH_M_PRE | SFH_M_MID | SFH_M_POST;
+using none = radices<1>;
+
+const sfh_fixed_kernel *table(int *count)
+{
+    static const sfh_fixed_kernel k[] = {
+        // complex<float> Bluestein convolution of length 20160 = 2^6 3^2 5 7 (n = 5004 .. 10080,
+        // e.g. the prime 10007): odd radices first, so that no power-of-two pass writes at a
+        // small ns (bank conflicts); 157.5 KiB of LDS, no room for the pad
+        entry<float, 1024, 4, BLUE, false, false, radices<9, 5, 7, 8, 8>, radices<8, 9, 5, 7, 8>>(),
+        // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 85 VGPRs, four workgroups per CU
+        entry<double, 384, 6, 0, false, true, radices<9, 5, 7, 8>, none>(),
+        // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB
+        // split tiles, two workgroups per CU
+        entry<double, 768, 6, 0, false, true, radices<9, 9, 9, 9>, none>(),
+        // complex<float> powers of two, pad layout where linear, split tiles
+        entry<float, 512, 4, 0, true, true, radices<8, 16, 8, 8>, none>(),
+        entry<float, 1024, 8, 0, true, true, radices<16, 16, 16, 4>, none>(),
+    };
+    *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
+    return k;
+}
+} // namespace
+
+extern "C" const sfh_fixed_kernel *sfh_fixed_kernels(int *count)
+{
+    return table(count);
+}

@@ -1,0 +1,432 @@
+// Length-specialized tile kernels: one transform of a fixed length L per workgroup, with
+// the whole radix program known at compile time.
+//
+// The generic tile kernel (fft_tile.hpp) reads its radix program, strides and twiddle
+// layout from the launch arguments, so every pass pays for run-time index arithmetic
+// (divisions by ns, two-level twiddle lookups in LDS and the products that rebuild
+// W^(j t)).  Here the lengths, radices, ns and butterfly counts are template constants:
+//   * butterfly indices, Stockham write positions and LDS offsets fold to constants
+//     (a pass costs one base address per butterfly),
+//   * pass twiddles come from a per-pass table in global memory (L2-resident, a few
+//     hundred KiB at most), laid out [t - 1][j] so lane j reads W_{ns R}^(j t) with one
+//     coalesced load per t: no VALU work and no LDS traffic for twiddles,
+//   * the first pass reads straight from HBM and the last writes straight to HBM, as in
+//     the generic kernel, with the same fused pre / mid / post operations, so a
+//     specialized launch is a drop-in for a generic launch of the same arguments.
+// Supported: one column per tile (W = 1), contiguous points (in_j = out_k = 1), fusion
+// modes plain (0) and Bluestein fused (PRE | MID | POST).  The second FFT of the
+// Bluestein program has its own radix order (its first radix must be the first FFT's last:
+// that pass is fused with the spectrum product in registers).
+#pragma once
+
+#include "fft_tile.hpp"
+
+namespace sfh
+{
+namespace fx
+{
+
+template <int... R>
+struct radices
+{
+    static constexpr int n = sizeof...(R);
+    static constexpr int v[sizeof...(R)] = {R...};
+};
+
+// product of the radices before pass p (the pass's ns)
+template <class RL>
+constexpr int ns_of(int p)
+{
+    int s = 1;
+    for (int q = 0; q < p; ++q) {
+        s *= RL::v[q];
+    }
+    return s;
+}
+
+template <class RL>
+constexpr int length_of()
+{
+    return ns_of<RL>(RL::n);
+}
+
+// twiddle entries of passes 1 .. p-1 (pass 0 has ns = 1 and needs none): pass q holds
+// ns_q (R_q - 1) values, [t - 1][j] = W_{ns_q R_q}^(j t)
+template <class RL>
+constexpr int tw_before(int p)
+{
+    int off = 0;
+    for (int q = 1; q < p; ++q) {
+        off += ns_of<RL>(q) * (RL::v[q] - 1);
+    }
+    return off;
+}
+
+enum
+{
+    IN_HBM = 0,
+    IN_SHARED = 1
+};
+enum
+{
+    OUT_SHARED = 0,
+    OUT_HBM = 1,
+    OUT_SPECTRUM = 2 // spectrum product, conjugation and the second FFT's first pass, then LDS
+};
+
+// One step of the fused program: pass `r` with `ns`, its inputs (HBM or LDS), its outputs
+// (LDS, HBM, or the spectrum product + second FFT's first pass), the ns of the pass whose
+// outputs it writes (1 after the spectrum product), its twiddle table offset, and the LDS
+// layout of its input and output boundaries (pad: point e at slot e + e / 32).
+struct step_info
+{
+    int r, ns, wns, in, out, twoff;
+    bool pad_in, pad_out;
+};
+
+// Program of the first FFT (radices R1) and, with MID, the second (R2, first pass fused
+// into the first FFT's last).  A boundary takes the pad layout when `pad` allows it and
+// both sides stay linear in the butterfly's point index t: the writer's points
+// base + t ns lie in one 32-point block (ns r divides 32) or ns is a multiple of 32, and
+// the reader's points i + t L / r have L / r a multiple of 32 or a divisor of 32.
+template <class R1, class R2, int MODE, bool PAD>
+struct program
+{
+    static constexpr bool two = (MODE & SFH_M_MID) != 0;
+    static constexpr int L = length_of<R1>();
+    static constexpr int count = two ? R1::n + R2::n - 1 : R1::n;
+
+    static constexpr step_info raw(int s)
+    {
+        step_info x{};
+        if (s < R1::n) {
+            x.r = R1::v[s];
+            x.ns = ns_of<R1>(s);
+            x.twoff = tw_before<R1>(s);
+            x.in = s == 0 ? IN_HBM : IN_SHARED;
+            x.out = s + 1 < R1::n ? OUT_SHARED : (two ? OUT_SPECTRUM : OUT_HBM);
+        } else {
+            const int p = s - R1::n + 1;
+            x.r = R2::v[p];
+            x.ns = ns_of<R2>(p);
+            x.twoff = tw_before<R1>(R1::n) + tw_before<R2>(p);
+            x.in = IN_SHARED;
+            x.out = s + 1 < count ? OUT_SHARED : OUT_HBM;
+        }
+        x.wns = x.out == OUT_SPECTRUM ? 1 : x.ns;
+        return x;
+    }
+    static constexpr bool writer_linear(const step_info &w)
+    {
+        return w.wns % 32 == 0 || (w.wns * w.r <= 32 && 32 % (w.wns * w.r) == 0);
+    }
+    static constexpr bool reader_linear(const step_info &r)
+    {
+        const int m = L / r.r;
+        return m % 32 == 0 || (m <= 32 && 32 % m == 0);
+    }
+    // boundary before step s (1 <= s < count)
+    static constexpr bool padded(int s)
+    {
+        return PAD && s >= 1 && s < count && writer_linear(raw(s - 1)) && reader_linear(raw(s));
+    }
+    static constexpr step_info step(int s)
+    {
+        step_info x = raw(s);
+        x.pad_in = padded(s);
+        x.pad_out = padded(s + 1);
+        return x;
+    }
+    // LDS points (slots) the tile needs
+    static constexpr int slots()
+    {
+        bool any = false;
+        for (int s = 1; s < count; ++s) {
+            any = any || padded(s);
+        }
+        return any ? L + (L - 1) / 32 + 1 : L;
+    }
+};
+
+// slot offset of point base + off inside a pad-linear run: off + off / 32
+template <bool PAD>
+constexpr uint32_t run_offset(int off)
+{
+    return static_cast<uint32_t>(PAD ? off + off / 32 : off);
+}
+template <bool PAD>
+__device__ __forceinline__ uint32_t base_slot(uint32_t e)
+{
+    return PAD ? e + (e >> 5) : e;
+}
+
+// Caller buffers are global memory: say so, so the compiler emits global (not flat)
+// accesses, whose completion counts do not mix with LDS operations.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SFH_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define SFH_GLOBAL_AS
+#endif
+template <typename T>
+__device__ __forceinline__ const SFH_GLOBAL_AS cx<T> *in_global(const cx<T> *p)
+{
+    return (const SFH_GLOBAL_AS cx<T> *)p;
+}
+template <typename T>
+__device__ __forceinline__ SFH_GLOBAL_AS cx<T> *out_global(cx<T> *p)
+{
+    return (SFH_GLOBAL_AS cx<T> *)p;
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ cx<T> load_point(const sfh_tile_args &a, const tile_ctx<T> &c, uint32_t j)
+{
+    if constexpr (MODE & SFH_M_PRE) {
+        const uint32_t p = j * a.pre_jmul + c.col * a.pre_cmul;
+        if (p >= a.nvalid_in) {
+            return {T(0), T(0)};
+        }
+        cx<T> v = in_global(c.src)[j];
+        if (a.flags & SFH_CONJ_IN) {
+            v.y = -v.y;
+        }
+        return cmul(v, static_cast<const cx<T> *>(a.pre)[p]);
+    } else {
+        cx<T> v = in_global(c.src)[j];
+        if (a.flags & SFH_CONJ_IN) {
+            v.y = -v.y;
+        }
+        return v;
+    }
+}
+
+// Butterfly counts of step S of program PROG: M = L / R butterflies, NB per work-item.
+template <int NT, int L, class PROG, int S>
+struct shape
+{
+    static constexpr step_info st = PROG::step(S);
+    static constexpr int R = st.r;
+    static constexpr int M = L / R;
+    static constexpr int NB = (M + NT - 1) / NT;
+    // butterfly q of a work-item exists (compile-time true except in a partial last round)
+    static __device__ __forceinline__ bool has(int q, uint32_t i)
+    {
+        return (q + 1) * NT <= M || i < static_cast<uint32_t>(M);
+    }
+};
+
+template <typename T, int NT, int L, class PROG, int S>
+struct regs
+{
+    using sh = shape<NT, L, PROG, S>;
+    cx<T> v[sh::NB][sh::R];
+};
+
+// Step S of a Stockham program on the work-item's butterflies i = tid + q NT (i < L/R):
+// inputs t of butterfly i are points i + t L/R; input t is multiplied by W_{NS R}^((i mod NS) t)
+// (steps that read LDS), the radix-R DFT runs, and output t goes to point
+// (i / NS) NS R + (i mod NS) + t NS of the next step.
+template <typename T, int NT, int L, int MODE, class PROG, int S>
+__device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T> &c, const cx<T> *tw0,
+                                        regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr step_info st = sh::st;
+    constexpr int R = sh::R, M = sh::M, NS = st.ns;
+    const cx<T> *tw = tw0 + st.twoff;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+            cx<T> *v = r.v[q];
+            if constexpr (st.in == IN_HBM) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    v[t] = load_point<T, MODE>(a, c, i + static_cast<uint32_t>(t * M));
+                }
+            } else if constexpr (NS > 1) {
+                const uint32_t j = i % static_cast<uint32_t>(NS);
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    v[t] = cmul(v[t], tw[(t - 1) * NS + j]);
+                }
+            }
+            dft<T, R>::run(v);
+            if constexpr (st.out == OUT_SPECTRUM) {
+                // natural-order outputs k = i + t M of the first FFT: times the spectrum,
+                // conjugated (the inverse FFT as conj(FFT(conj))), then the second FFT's first
+                // pass (radix R, ns = 1) on the same points
+                const cx<T> *mid = static_cast<const cx<T> *>(a.mid);
+                const uint32_t s0 = i * a.mid_kmul + c.col * a.mid_cmul;
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    v[t] = cconj(cmul(v[t], mid[s0 + static_cast<uint32_t>(t * M) * a.mid_kmul]));
+                }
+                dft<T, R>::run(v);
+            }
+        }
+    }
+}
+
+// Last step: output t of butterfly i is point i + t M (NS R = L, i < NS): post-op, epilogue, HBM.
+template <typename T, int NT, int L, int MODE, class PROG, int S>
+__device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> &c, const regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr int R = sh::R, M = sh::M;
+    const epilogue<T> ep(a);
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const uint32_t k = i + static_cast<uint32_t>(t * M);
+                if constexpr (MODE & SFH_M_POST) {
+                    const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
+                    if (pq < a.nvalid_out) {
+                        out_global(c.dst)[k] = ep(cmul(r.v[q][t], static_cast<const cx<T> *>(a.post)[pq]));
+                    }
+                } else {
+                    out_global(c.dst)[k] = ep(r.v[q][t]);
+                }
+            }
+        }
+    }
+}
+
+// Component access of one point: the whole complex value (PART = 2), or its real (0) or
+// imaginary (1) part alone for the split exchange.
+template <int PART, typename T, typename U>
+__device__ __forceinline__ void put(U *lds, uint32_t slot, const cx<T> &v)
+{
+    if constexpr (PART == 2) {
+        reinterpret_cast<cx<T> *>(lds)[slot] = v;
+    } else {
+        reinterpret_cast<T *>(lds)[slot] = PART == 0 ? v.x : v.y;
+    }
+}
+template <int PART, typename T, typename U>
+__device__ __forceinline__ void get(const U *lds, uint32_t slot, cx<T> &v)
+{
+    if constexpr (PART == 2) {
+        v = reinterpret_cast<const cx<T> *>(lds)[slot];
+    } else if constexpr (PART == 0) {
+        v.x = reinterpret_cast<const T *>(lds)[slot];
+    } else {
+        v.y = reinterpret_cast<const T *>(lds)[slot];
+    }
+}
+
+// Write step S's outputs (component PART) to their LDS slots.
+template <int PART, typename T, int NT, int L, class PROG, int S>
+__device__ __forceinline__ void write_out(unsigned char *lds, const regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr step_info st = sh::st;
+    constexpr int R = sh::R, WNS = st.wns;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+            const uint32_t kq = i / static_cast<uint32_t>(WNS);
+            const uint32_t b = base_slot<st.pad_out>(kq * static_cast<uint32_t>(WNS * (R - 1)) + i);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                put<PART>(lds, b + run_offset<st.pad_out>(t * WNS), r.v[q][t]);
+            }
+        }
+    }
+}
+
+// Read step S's inputs (component PART): points i + t M.
+template <int PART, typename T, int NT, int L, class PROG, int S>
+__device__ __forceinline__ void read_in(const unsigned char *lds, regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr step_info st = sh::st;
+    constexpr int R = sh::R, M = sh::M;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+            const uint32_t b = base_slot<st.pad_in>(i);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                get<PART>(lds, b + run_offset<st.pad_in>(t * M), r.v[q][t]);
+            }
+        }
+    }
+}
+
+// LDS exchange from step S's outputs to step S + 1's inputs.  Every work-item's values are
+// in registers on both sides, so with SPLIT the real parts move first and the imaginary
+// parts second through an LDS buffer of one real per point (half the footprint, twice the
+// barriers).  The leading barrier ends the previous exchange's reads (none before the first).
+template <bool SPLIT, typename T, int NT, int L, class PROG, int S>
+__device__ __forceinline__ void exchange(unsigned char *lds, const regs<T, NT, L, PROG, S> &out,
+                                         regs<T, NT, L, PROG, S + 1> &in)
+{
+    if constexpr (S > 0) {
+        __syncthreads();
+    }
+    if constexpr (SPLIT) {
+        write_out<0>(lds, out);
+        __syncthreads();
+        read_in<0>(lds, in);
+        __syncthreads();
+        write_out<1>(lds, out);
+        __syncthreads();
+        read_in<1>(lds, in);
+    } else {
+        write_out<2>(lds, out);
+        __syncthreads();
+        read_in<2>(lds, in);
+    }
+}
+
+template <bool SPLIT, typename T, int NT, int L, int MODE, class PROG, int S>
+__device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<T> &c, unsigned char *lds,
+                                         const cx<T> *tw, regs<T, NT, L, PROG, S> &r)
+{
+    compute<T, NT, L, MODE, PROG, S>(a, c, tw, r);
+    if constexpr (S + 1 == PROG::count) {
+        store<T, NT, L, MODE, PROG, S>(a, c, r);
+    } else {
+        regs<T, NT, L, PROG, S + 1> next;
+        exchange<SPLIT>(lds, r, next);
+        run_from<SPLIT, T, NT, L, MODE, PROG, S + 1>(a, c, lds, tw, next);
+    }
+}
+
+} // namespace fx
+
+// Entry: tile g = blockIdx.x is one transform (W = 1), addressed as by the generic kernel.
+// WAVES: register budget (waves per SIMD); PAD: the pad layout may be used where linear;
+// SPLIT: LDS exchanges move real and imaginary parts separately (half the LDS per tile).
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, class R1, class R2>
+__global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(WAVES)
+void fixed_fft(const sfh_tile_args a_byval)
+{
+    using prog = fx::program<R1, R2, MODE, PAD>;
+    constexpr int L = prog::L;
+    static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
+                  "second FFT: same length, first radix = the first FFT's last");
+#if defined(__HIP_DEVICE_COMPILE__)
+    const sfh_tile_args &a = *(const sfh_tile_args *)(__builtin_amdgcn_kernarg_segment_ptr());
+    (void)a_byval;
+#else
+    const sfh_tile_args &a = a_byval;
+#endif
+    SFH_DYNAMIC_LDS(sfh_smem);
+    tile_ctx<T> c;
+    set_tile<T, MODE>(a, c, blockIdx.x, threadIdx.x);
+    fx::regs<T, NT, L, prog, 0> r;
+    fx::run_from<SPLIT, T, NT, L, MODE, prog, 0>(a, c, sfh_smem, static_cast<const cx<T> *>(a.twtab), r);
+}
+
+} // namespace sfh

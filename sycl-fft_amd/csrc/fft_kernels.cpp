@@ -157,3 +157,27 @@ extern "C" int sfh_launch_queue(
                                             dim3(static_cast<uint32_t>(variant->nt)), kernel_args, lds_bytes,
                                             stream));
 }
+
+extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_args *args, hipStream_t stream)
+{
+    if (args->ntiles == 0) {
+        return 0;
+    }
+    if (args->ntiles > 0xffffffffull / static_cast<uint64_t>(kernel->nt) || args->W != 1 || args->L != kernel->L
+        || args->lds_bytes > 163840) {
+        return static_cast<int>(hipErrorInvalidConfiguration);
+    }
+    static std::once_flag lds_once;
+    std::call_once(lds_once, [] {
+        int n = 0;
+        const sfh_fixed_kernel *k = sfh_fixed_kernels(&n);
+        for (int i = 0; i < n; ++i) {
+            (void)hipFuncSetAttribute(k[i].fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        }
+        (void)hipGetLastError();
+    });
+    void *kernel_args[] = {const_cast<sfh_tile_args *>(args)};
+    return static_cast<int>(hipLaunchKernel(kernel->fn, dim3(static_cast<uint32_t>(args->ntiles)),
+                                            dim3(static_cast<uint32_t>(kernel->nt)), kernel_args, args->lds_bytes,
+                                            stream));
+}

@@ -33,6 +33,25 @@ struct sfh_kernel_variant
     int tw_global;                 // 1: twiddle tables read from global memory, none in LDS
 };
 
+// Length-specialized tile kernels (fft_fixed.hip): one transform of length L per workgroup
+// of nt work-items, radix program r1 (and r2 for the Bluestein mode's second FFT) fixed at
+// compile time; arguments as for the generic entry of `mode_index`, except twtab: the
+// per-pass twiddle tables of the program ([t - 1][j] = W_{ns r}^(j t) for passes 1.. of r1,
+// then of r2), and lds_bytes = the entry's own.
+struct sfh_fixed_kernel
+{
+    const void *fn;
+    int is_double;
+    int mode_index; // SFH_MI_PLAIN or SFH_MI_BLUE_FUSED
+    int nt;
+    uint32_t L;
+    uint32_t lds_bytes;
+    int n1, n2;
+    int r1[8], r2[8];
+};
+extern "C" const sfh_fixed_kernel *sfh_fixed_kernels(int *count);
+extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_args *args, hipStream_t stream);
+
 extern "C" int sfh_device_cus();
 
 extern "C" const sfh_kernel_variant *sfh_variants(int is_double, int *count);

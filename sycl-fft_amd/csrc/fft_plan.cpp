@@ -214,6 +214,11 @@ struct launch
     uint32_t unit_dim{0};
     uint64_t unit_extent{0};
     int64_t unit_in_stride{0}, unit_out_stride{0};
+    // length-specialized kernel replacing the generic entry (null: generic), with its
+    // per-pass twiddle tables (offset into the plan's table block, then device address)
+    const sfh_fixed_kernel *fixed{nullptr};
+    int64_t fx_tw_off{-1};
+    const void *fx_twtab{nullptr};
     std::string desc;
 };
 
@@ -1541,6 +1546,100 @@ static void release(plan *p)
 // trail > 1: every transform carries an untransformed innermost dimension of that extent
 // (C order [batch][lengths...][trail]); the axes are strided by it.  Multi-D real plans use
 // it for the leading axes of the half spectrum [..][n/2 + 1].
+// Per-pass twiddle tables of a length-specialized kernel: for every pass q >= 1 of its first
+// radix program (then of its second), [t - 1][j] = W_{ns r}^(j t), t < r, j < ns.
+static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k)
+{
+    std::string key = "fx" + std::to_string(k.L);
+    std::vector<cxl> v;
+    const auto add = [&](const int *r, int n) {
+        uint64_t ns = static_cast<uint64_t>(r[0]);
+        key += "_";
+        for (int q = 0; q < n; ++q) {
+            key += std::to_string(r[q]) + ".";
+        }
+        for (int q = 1; q < n; ++q) {
+            const uint64_t R = static_cast<uint64_t>(r[q]);
+            for (uint64_t t = 1; t < R; ++t) {
+                for (uint64_t j = 0; j < ns; ++j) {
+                    v.push_back(root_ld(j * t, ns * R));
+                }
+            }
+            ns *= R;
+        }
+    };
+    add(k.r1, k.n1);
+    if (k.n2) {
+        add(k.r2, k.n2);
+    }
+    return table(p, key, v);
+}
+
+// Length-specialized kernels (fft_fixed.hip) for launches that fit their contract: one
+// column per tile, contiguous points, no LDS load/store loops, one tile per workgroup,
+// plain or Bluestein-fused mode, VALU precision.  SYCLFFT_HIP_FIXED=0 keeps every launch on
+// the generic kernel.
+static void choose_fixed(plan &p)
+{
+    if (const char *fe = std::getenv("SYCLFFT_HIP_FIXED")) {
+        if (std::strcmp(fe, "0") == 0) {
+            return;
+        }
+        if (std::strcmp(fe, "1") != 0) {
+            throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "SYCLFFT_HIP_FIXED is not 0 or 1");
+        }
+    }
+    if (p.cfg.precision != SYCLFFT_HIP_PRECISION_VALU) {
+        return;
+    }
+    int nk = 0;
+    const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    for (auto &l : p.launches) {
+        const sfh_tile_args &a = l.args;
+        if (a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0
+            || l.grid < a.ntiles || (l.mode != SFH_MI_PLAIN && l.mode != SFH_MI_BLUE_FUSED)) {
+            continue;
+        }
+        for (int i = 0; i < nk; ++i) {
+            const sfh_fixed_kernel &k = ks[i];
+            if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L
+                && k.lds_bytes <= k_lds_limit) {
+                l.fixed = &k;
+                l.fx_tw_off = fixed_tables(p, k);
+                l.desc += " fixed=" + std::to_string(k.nt) + "x" + std::to_string(k.L);
+                break;
+            }
+        }
+    }
+}
+
+// Back to the generic kernel for a launch whose arguments leave the specialized contract.
+static void unfix(launch &l)
+{
+    if (l.fixed) {
+        l.fixed = nullptr;
+        const size_t at = l.desc.find(" fixed=");
+        if (at != std::string::npos) {
+            const size_t end = l.desc.find(' ', at + 1);
+            l.desc.erase(at, end == std::string::npos ? std::string::npos : end - at);
+        }
+    }
+}
+
+// One launch of l with arguments a (buffers and tile counts already set).
+static hipError_t launch_tile(const plan &p, const launch &l, sfh_tile_args &a, uint32_t grid, hipStream_t stream)
+{
+    if (l.fixed) {
+        if (a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0) {
+            return hipErrorInvalidConfiguration; // outside the specialized kernel's contract
+        }
+        a.twtab = l.fx_twtab;
+        a.lds_bytes = l.fixed->lds_bytes;
+        return static_cast<hipError_t>(sfh_launch_fixed(l.fixed, &a, stream));
+    }
+    return static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, grid, stream));
+}
+
 static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tuning &tune, bool batch_inner = false,
                         uint64_t trail = 1)
 {
@@ -1707,6 +1806,7 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
             l.desc += " tw=global lds=" + std::to_string(data_bytes);
         }
     }
+    choose_fixed(*p);
     if (!p->host_tables.empty()) {
         hip_check(hipMalloc(&p->tables, p->host_tables.size()), "hipMalloc(tables)");
         hip_check(
@@ -1729,6 +1829,7 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
         l.args.pre = ptr(l.pre_off);
         l.args.mid = ptr(l.mid_off);
         l.args.post = ptr(l.post_off);
+        l.fx_twtab = ptr(l.fx_tw_off);
     }
     p->host_tables.clear();
     p->host_tables.shrink_to_fit();
@@ -2110,9 +2211,7 @@ static void execute_checked(
         sfh_tile_args a = l.args;
         a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
         a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
-        hip_check(
-            static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, l.grid, stream)),
-            "tile kernel launch");
+        hip_check(launch_tile(p, l, a, l.grid, stream), "tile kernel launch");
     }
     if (!capturing) {
         hip_check(hipEventRecord(p.last, stream), "hipEventRecord(last)");
@@ -2315,10 +2414,8 @@ static void execute_units(
         }
         const bool persistent = l.mode == SFH_MI_PLAIN_PF || l.mode == SFH_MI_T4_PF;
         const uint64_t grid = persistent ? std::min<uint64_t>(l.grid, a.ntiles) : a.ntiles;
-        hip_check(
-            static_cast<hipError_t>(sfh_launch_tile(
-                &p.variants[l.variant], l.mode, &a, static_cast<uint32_t>(std::min<uint64_t>(grid, 0xffffffffu)), stream)),
-            "tile kernel launch");
+        hip_check(launch_tile(p, l, a, static_cast<uint32_t>(std::min<uint64_t>(grid, 0xffffffffu)), stream),
+                  "tile kernel launch");
     }
     if (!affine) {
         hip_check(hipEventRecord(p.unit_free[slot], stream), "hipEventRecord(unit slot)");
@@ -2367,8 +2464,10 @@ static queue_ctx &queue_context(int dev)
 static bool queueable(const plan &p)
 {
     // the queue launch adds a 16-byte item slot behind the largest tile
+    // (a plan on a length-specialized kernel runs its own launch: the queue form is the
+    // generic program, whose results may differ from the plan's in the last bit)
     return !p.copy_only && p.launches.size() == 1 && p.launches[0].mode == SFH_MI_PLAIN
-           && p.variants[p.launches[0].variant].fn[SFH_MI_QUEUE] != nullptr
+           && !p.launches[0].fixed && p.variants[p.launches[0].variant].fn[SFH_MI_QUEUE] != nullptr
            && p.launches[0].args.lds_bytes + 32u <= k_lds_limit;
 }
 
@@ -2832,6 +2931,9 @@ static plan *create_layout_plan(const syclfft_hip_plan_config_v1 *cfg, const syc
                 a.in_j = static_cast<int32_t>(lay->in_stride);
                 a.out_w = static_cast<int32_t>(lay->out_distance);
                 a.out_k = static_cast<int32_t>(lay->out_stride);
+                if (a.in_j != 1 || a.out_k != 1) {
+                    unfix(l); // strided points: the generic kernel
+                }
                 l.desc += " layout(in " + std::to_string(lay->in_stride) + "/" + std::to_string(lay->in_distance)
                           + ", out " + std::to_string(lay->out_stride) + "/" + std::to_string(lay->out_distance) + ")";
                 fused = true;

@@ -7,7 +7,9 @@ sentinel, runs the plan through the C ABI, and checks
   * that no gap of the output layout was written (out of place);
   * bit-equality with the contiguous plan of the same config on the same values: the fused
     path runs that plan's own launch with other strides, and the gather / scatter path
-    copies exactly around the contiguous in-place plan.
+    copies exactly around the contiguous in-place plan.  Where the contiguous plan runs a
+    length-specialized kernel (contiguous points only) and the fused strided launch the
+    generic one, the two agree within the oracle bound instead.
 Interleaved batches (stride = batch, distance 1, both sides) are planned as strided axes over
 the caller's buffers (any rank and plan form; checked against the oracle only, since their
 plan forms differ from the contiguous plan's); other rank-1 interleaved layouts over one
@@ -75,9 +77,11 @@ def gaps_kept(host, off):
 
 
 def contiguous_result(lengths, batch, direction, double, norm, x, in_place):
+    """The contiguous plan's result and whether it ran a length-specialized kernel."""
     dt = torch.complex128 if double else torch.complex64
     xd = torch.from_numpy(x.reshape(-1).astype(np.complex128 if double else np.complex64)).to("cuda")
     with sh.Plan(lengths, batch, direction, double=double, in_place=in_place, normalization=norm) as p:
+        fixed = "fixed=" in p.describe()
         if in_place:
             p.execute(xd)
             yd = xd
@@ -85,7 +89,7 @@ def contiguous_result(lengths, batch, direction, double, norm, x, in_place):
             yd = torch.empty_like(xd, dtype=dt)
             p.execute(xd, yd)
         torch.cuda.synchronize()
-    return yd.cpu().numpy()
+    return yd.cpu().numpy(), fixed
 
 
 # (lengths, batch, (istride, idist, ostride, odist, iplanar, oplanar), in_place, fusable)
@@ -152,9 +156,14 @@ def test_layout_vs_oracle(lengths, batch, lay, in_place, fused, double):
         assert gaps_kept(host, ooff), "a gap of the output layout was written"
     if "batch-inner" in desc:
         return  # planned as strided axes: other plan forms than the contiguous plan's
-    # the fused path runs the contiguous plan's launch; the gather path its in-place plan
-    contiguous = contiguous_result(lengths, batch, direction, double, norm, x, in_place or not is_fused)
-    assert np.array_equal(got.reshape(-1), contiguous), "differs from the contiguous plan"
+    # the fused path runs the contiguous plan's launch; the gather path its in-place plan.
+    # A contiguous plan on a length-specialized kernel (contiguous points only) and a fused
+    # strided launch of the generic kernel round differently: then within the bound.
+    contiguous, fixed = contiguous_result(lengths, batch, direction, double, norm, x, in_place or not is_fused)
+    if fixed and is_fused and "fixed=" not in desc:
+        assert rel_l2(got.reshape(-1), contiguous) <= tol
+    else:
+        assert np.array_equal(got.reshape(-1), contiguous), "differs from the contiguous plan"
 
 
 @pytest.mark.parametrize("n,batch,lay,fused", [
