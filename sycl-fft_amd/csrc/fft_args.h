@@ -46,7 +46,11 @@ enum sfh_mode : int
     // fp32 plans: radix-16 passes as 16-point DFT matrix products on MFMA with split
     // operands (hi + lo, three products, FP32 accumulation) instead of VALU butterflies
     SFH_M_SPLIT_BF16 = 16, // BF16 hi/lo
-    SFH_M_SPLIT_F16 = 32   // FP16 hi/lo, each butterfly scaled by a power of two
+    SFH_M_SPLIT_F16 = 32,  // FP16 hi/lo, each butterfly scaled by a power of two
+    // length-specialized Bluestein over two workgroups per transform (with PRE | MID | POST):
+    // each writes its half's partial result to pair_buf, and the second of the pair to finish
+    // adds the two and writes the output
+    SFH_M_PAIR = 64
 };
 
 // Run-time flags.
@@ -132,6 +136,11 @@ struct sfh_tile_args
     uint32_t pad_store;           // LDS layout of the tile the STORE_LDS loop reads
     uint32_t lds_bytes;           // dynamic LDS of the launch
     double out_sx, out_sy; // store epilogue: re * out_sx, im * out_sy
+    // SFH_M_PAIR: partial results, slot (2 t + h) of pair_stride points for half h of transform
+    // t, and one arrival counter per transform (even before an execution: each adds 2)
+    void *pair_buf;
+    uint32_t *pair_count;
+    uint32_t pair_stride;
     sfh_pass pass[SFH_MAX_PASSES];
 };
 

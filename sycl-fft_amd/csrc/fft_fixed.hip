@@ -10,14 +10,14 @@ namespace
 {
 using sfh::fx::radices;
 
-template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, class R1, class R2>
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2>
 sfh_fixed_kernel entry()
 {
     using prog = sfh::fx::program<R1, R2, MODE, PAD>;
     sfh_fixed_kernel k{};
-    k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, R1, R2>);
+    k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, PF, R1, R2>);
     k.is_double = sizeof(T) == 8;
-    k.mode_index = (MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED : SFH_MI_PLAIN;
+    k.mode_index = (MODE & SFH_M_PAIR) ? SFH_FX_BLUE_PAIR : ((MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED : SFH_MI_PLAIN);
     k.nt = NT;
     k.L = static_cast<uint32_t>(prog::L);
     // LDS bytes per tile: one complex value per slot, or one real with the split exchange
@@ -42,15 +42,18 @@ const sfh_fixed_kernel *table(int *count)
         // complex<float> Bluestein convolution of length 20160 = 2^6 3^2 5 7 (n = 5004 .. 10080,
         // e.g. the prime 10007): odd radices first, so that no power-of-two pass writes at a
         // small ns (bank conflicts); 157.5 KiB of LDS, no room for the pad
-        entry<float, 1024, 4, BLUE, false, false, radices<9, 5, 7, 8, 8>, radices<8, 9, 5, 7, 8>>(),
+        entry<float, 1024, 4, BLUE, false, false, false, radices<9, 5, 7, 8, 8>, radices<8, 9, 5, 7, 8>>(),
+        // the same convolution over a pair of workgroups per transform: half-length FFTs of
+        // 10080 = 2^5 3^2 5 7 (n <= 10080), for batches too small to fill the device
+        entry<float, 1024, 4, BLUE | SFH_M_PAIR, false, false, true, radices<9, 5, 7, 4, 8>, radices<8, 9, 5, 7, 4>>(),
         // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 85 VGPRs, four workgroups per CU
-        entry<double, 384, 6, 0, false, true, radices<9, 5, 7, 8>, none>(),
+        entry<double, 384, 6, 0, false, true, false, radices<9, 5, 7, 8>, none>(),
         // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB
         // split tiles, two workgroups per CU
-        entry<double, 768, 6, 0, false, true, radices<9, 9, 9, 9>, none>(),
+        entry<double, 768, 6, 0, false, true, false, radices<9, 9, 9, 9>, none>(),
         // complex<float> powers of two, pad layout where linear, split tiles
-        entry<float, 512, 4, 0, true, true, radices<8, 16, 8, 8>, none>(),
-        entry<float, 1024, 8, 0, true, true, radices<16, 16, 16, 4>, none>(),
+        entry<float, 512, 4, 0, true, true, false, radices<8, 16, 8, 8>, none>(),
+        entry<float, 1024, 8, 0, true, true, false, radices<16, 16, 16, 4>, none>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

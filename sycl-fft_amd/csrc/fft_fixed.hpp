@@ -14,7 +14,8 @@
 //     the generic kernel, with the same fused pre / mid / post operations, so a
 //     specialized launch is a drop-in for a generic launch of the same arguments.
 // Supported: one column per tile (W = 1), contiguous points (in_j = out_k = 1), fusion
-// modes plain (0) and Bluestein fused (PRE | MID | POST).  The second FFT of the
+// modes plain (0), Bluestein fused (PRE | MID | POST) and Bluestein over a pair of
+// workgroups per transform (PRE | MID | POST | PAIR).  The second FFT of the
 // Bluestein program has its own radix order (its first radix must be the first FFT's last:
 // that pass is fused with the spectrum product in registers).
 #pragma once
@@ -167,6 +168,13 @@ __device__ __forceinline__ uint32_t base_slot(uint32_t e)
 #else
 #define SFH_GLOBAL_AS
 #endif
+#if defined(__HIP__)
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt at their maxima): every vector
+// memory access of the wave, stores included, has completed
+#define SFH_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)
+#else
+#define SFH_WAIT_VMEM() __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#endif
 template <typename T>
 __device__ __forceinline__ const SFH_GLOBAL_AS cx<T> *in_global(const cx<T> *p)
 {
@@ -222,18 +230,50 @@ struct regs
     cx<T> v[sh::NB][sh::R];
 };
 
+// Pass twiddles of step S for the work-item's butterflies, w[q][t - 1] = W_{NS R}^((i mod NS) t).
+// They are loaded one step ahead (before the previous step's butterflies and LDS exchange),
+// so their L2 latency overlaps that work instead of stalling every wave after the barrier.
+template <typename T, int NT, int L, class PROG, int S>
+struct twregs
+{
+    using sh = shape<NT, L, PROG, S>;
+    static constexpr bool used = sh::st.in == IN_SHARED && sh::st.ns > 1;
+    cx<T> w[used ? sh::NB : 1][used ? sh::R - 1 : 1];
+};
+
+template <typename T, int NT, int L, class PROG, int S>
+__device__ __forceinline__ void load_twiddles(const cx<T> *tw0, twregs<T, NT, L, PROG, S> &w)
+{
+    using sh = shape<NT, L, PROG, S>;
+    if constexpr (twregs<T, NT, L, PROG, S>::used) {
+        constexpr int R = sh::R, NS = sh::st.ns;
+        const cx<T> *tw = tw0 + sh::st.twoff;
+        const uint32_t tid = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < sh::NB; ++q) {
+            const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+            if (sh::has(q, i)) {
+                const uint32_t j = i % static_cast<uint32_t>(NS);
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    w.w[q][t - 1] = tw[(t - 1) * NS + j];
+                }
+            }
+        }
+    }
+}
+
 // Step S of a Stockham program on the work-item's butterflies i = tid + q NT (i < L/R):
 // inputs t of butterfly i are points i + t L/R; input t is multiplied by W_{NS R}^((i mod NS) t)
 // (steps that read LDS), the radix-R DFT runs, and output t goes to point
 // (i / NS) NS R + (i mod NS) + t NS of the next step.
-template <typename T, int NT, int L, int MODE, class PROG, int S>
+template <bool PF, typename T, int NT, int L, int MODE, class PROG, int S>
 __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T> &c, const cx<T> *tw0,
-                                        regs<T, NT, L, PROG, S> &r)
+                                        const twregs<T, NT, L, PROG, S> &w, regs<T, NT, L, PROG, S> &r)
 {
     using sh = shape<NT, L, PROG, S>;
     constexpr step_info st = sh::st;
     constexpr int R = sh::R, M = sh::M, NS = st.ns;
-    const cx<T> *tw = tw0 + st.twoff;
     const uint32_t tid = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < sh::NB; ++q) {
@@ -246,10 +286,12 @@ __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T
                     v[t] = load_point<T, MODE>(a, c, i + static_cast<uint32_t>(t * M));
                 }
             } else if constexpr (NS > 1) {
+                // prefetched (PF) or loaded here, per butterfly (fewer live registers)
                 const uint32_t j = i % static_cast<uint32_t>(NS);
+                const cx<T> *tw = tw0 + st.twoff;
 #pragma unroll
                 for (int t = 1; t < R; ++t) {
-                    v[t] = cmul(v[t], tw[(t - 1) * NS + j]);
+                    v[t] = cmul(v[t], PF ? w.w[q][t - 1] : tw[(t - 1) * NS + j]);
                 }
             }
             dft<T, R>::run(v);
@@ -264,6 +306,97 @@ __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T
                     v[t] = cconj(cmul(v[t], mid[s0 + static_cast<uint32_t>(t * M) * a.mid_kmul]));
                 }
                 dft<T, R>::run(v);
+            }
+        }
+    }
+}
+
+// Point access that is coherent across the device without cache maintenance: agent-scope
+// relaxed atomic accesses, which gfx950 performs with the sc1 (coherent) cache policy, so a
+// partial result written on one XCD is read from memory, not from a stale L2 line, on
+// another.  (A device-scope fence would write back and invalidate whole L2 caches instead:
+// measured at ~1 us per workgroup.)
+template <typename T>
+__device__ __forceinline__ void coherent_put(cx<T> *p, const cx<T> &v)
+{
+    static_assert(sizeof(cx<T>) % 8 == 0, "complex of 32- or 64-bit parts");
+    uint64_t w[sizeof(cx<T>) / 8];
+    __builtin_memcpy(w, &v, sizeof(w));
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(sizeof(cx<T>) / 8); ++k) {
+        __hip_atomic_store(reinterpret_cast<uint64_t *>(p) + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+template <typename T>
+__device__ __forceinline__ cx<T> coherent_get(const cx<T> *p)
+{
+    uint64_t w[sizeof(cx<T>) / 8];
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(sizeof(cx<T>) / 8); ++k) {
+        w[k] = __hip_atomic_load(reinterpret_cast<const uint64_t *>(p) + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    cx<T> v;
+    __builtin_memcpy(&v, w, sizeof(v));
+    return v;
+}
+
+// Two workgroups per transform (SFH_M_PAIR): half h of transform b holds its partial result
+// P_h[k] = v[k] post[2k + h] (k < n) in registers.  Both halves store theirs to slot 2b + h
+// of pair_buf with coherent stores, wait until every store of the workgroup has completed
+// (each wave waits for its stores, then the barrier), and count their arrival on
+// pair_count[b]; the half that arrives second (odd count: each execution adds 2) reads the
+// partner's slot with coherent loads, issued only after it has seen the count, and writes
+// ep(P_0 + P_1) to the output.  The sum is the same whichever half adds it (floating-point
+// addition commutes).  Nothing waits on another workgroup, so no arrival order can stall.
+template <typename T, int NT, int L, int MODE, class PROG, int S>
+__device__ __forceinline__ void store_pair(const sfh_tile_args &a, const tile_ctx<T> &c, unsigned char *lds,
+                                           regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr int R = sh::R, M = sh::M;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t b = c.w0, h = c.blo;
+    cx<T> *const pair = static_cast<cx<T> *>(a.pair_buf);
+    cx<T> *const mine = pair + (2 * static_cast<uint64_t>(b) + h) * a.pair_stride;
+    const cx<T> *const other = pair + (2 * static_cast<uint64_t>(b) + (h ^ 1u)) * a.pair_stride;
+    const cx<T> *post = static_cast<const cx<T> *>(a.post);
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const uint32_t k = i + static_cast<uint32_t>(t * M);
+                const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
+                if (pq < a.nvalid_out) {
+                    r.v[q][t] = cmul(r.v[q][t], post[pq]);
+                    coherent_put(mine + k, r.v[q][t]);
+                }
+            }
+        }
+    }
+    SFH_WAIT_VMEM(); // this wave's stores have completed
+    __syncthreads(); // ... every wave's (and the last exchange's LDS reads are done)
+    uint32_t *const flag = reinterpret_cast<uint32_t *>(lds);
+    if (tid == 0) {
+        *flag = __hip_atomic_fetch_add(a.pair_count + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
+    }
+    __syncthreads();
+    if (*flag == 0u) {
+        return; // first of the pair: the partner adds
+    }
+    const epilogue<T> ep(a);
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const uint32_t k = i + static_cast<uint32_t>(t * M);
+                const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
+                if (pq < a.nvalid_out) {
+                    out_global(c.dst)[k] = ep(r.v[q][t] + coherent_get(other + k));
+                }
             }
         }
     }
@@ -389,17 +522,27 @@ __device__ __forceinline__ void exchange(unsigned char *lds, const regs<T, NT, L
     }
 }
 
-template <bool SPLIT, typename T, int NT, int L, int MODE, class PROG, int S>
+template <bool SPLIT, bool PF, typename T, int NT, int L, int MODE, class PROG, int S>
 __device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<T> &c, unsigned char *lds,
-                                         const cx<T> *tw, regs<T, NT, L, PROG, S> &r)
+                                         const cx<T> *tw, regs<T, NT, L, PROG, S> &r,
+                                         const twregs<T, NT, L, PROG, S> &w)
 {
-    compute<T, NT, L, MODE, PROG, S>(a, c, tw, r);
     if constexpr (S + 1 == PROG::count) {
-        store<T, NT, L, MODE, PROG, S>(a, c, r);
+        compute<PF, T, NT, L, MODE, PROG, S>(a, c, tw, w, r);
+        if constexpr ((MODE & SFH_M_PAIR) != 0) {
+            store_pair<T, NT, L, MODE, PROG, S>(a, c, lds, r);
+        } else {
+            store<T, NT, L, MODE, PROG, S>(a, c, r);
+        }
     } else {
+        twregs<T, NT, L, PROG, S + 1> wn;
+        if constexpr (PF) {
+            load_twiddles(tw, wn); // one step ahead
+        }
+        compute<PF, T, NT, L, MODE, PROG, S>(a, c, tw, w, r);
         regs<T, NT, L, PROG, S + 1> next;
         exchange<SPLIT>(lds, r, next);
-        run_from<SPLIT, T, NT, L, MODE, PROG, S + 1>(a, c, lds, tw, next);
+        run_from<SPLIT, PF, T, NT, L, MODE, PROG, S + 1>(a, c, lds, tw, next, wn);
     }
 }
 
@@ -407,8 +550,9 @@ __device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<
 
 // Entry: tile g = blockIdx.x is one transform (W = 1), addressed as by the generic kernel.
 // WAVES: register budget (waves per SIMD); PAD: the pad layout may be used where linear;
-// SPLIT: LDS exchanges move real and imaginary parts separately (half the LDS per tile).
-template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, class R1, class R2>
+// SPLIT: LDS exchanges move real and imaginary parts separately (half the LDS per tile);
+// PF: pass twiddles are loaded one step ahead (needs registers for them across a step).
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2>
 __global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(WAVES)
 void fixed_fft(const sfh_tile_args a_byval)
 {
@@ -426,7 +570,8 @@ void fixed_fft(const sfh_tile_args a_byval)
     tile_ctx<T> c;
     set_tile<T, MODE>(a, c, blockIdx.x, threadIdx.x);
     fx::regs<T, NT, L, prog, 0> r;
-    fx::run_from<SPLIT, T, NT, L, MODE, prog, 0>(a, c, sfh_smem, static_cast<const cx<T> *>(a.twtab), r);
+    const fx::twregs<T, NT, L, prog, 0> w0{}; // the first step reads HBM: no pass twiddles
+    fx::run_from<SPLIT, PF, T, NT, L, MODE, prog, 0>(a, c, sfh_smem, static_cast<const cx<T> *>(a.twtab), r, w0);
 }
 
 } // namespace sfh

@@ -38,11 +38,15 @@ struct sfh_kernel_variant
 // compile time; arguments as for the generic entry of `mode_index`, except twtab: the
 // per-pass twiddle tables of the program ([t - 1][j] = W_{ns r}^(j t) for passes 1.. of r1,
 // then of r2), and lds_bytes = the entry's own.
+// mode_index of the length-specialized Bluestein form over two workgroups per transform
+// (no generic counterpart)
+#define SFH_FX_BLUE_PAIR 64
+
 struct sfh_fixed_kernel
 {
     const void *fn;
     int is_double;
-    int mode_index; // SFH_MI_PLAIN or SFH_MI_BLUE_FUSED
+    int mode_index; // SFH_MI_PLAIN, SFH_MI_BLUE_FUSED or SFH_FX_BLUE_PAIR
     int nt;
     uint32_t L;
     uint32_t lds_bytes;

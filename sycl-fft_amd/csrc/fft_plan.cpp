@@ -281,6 +281,9 @@ struct plan
     void *tables{};
     uint64_t scratch_elems{};
     void *scratch{};
+    // Bluestein pair launches: one arrival counter per transform (zeroed at creation)
+    uint64_t pair_transforms{};
+    uint32_t *pair_counts{};
     hipEvent_t last{};
     bool have_last{};
     hipStream_t last_stream{}; // stream of the last execution
@@ -1071,6 +1074,9 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
 // writes P_h[m] = c[m] E_0[m] and c[m] W_M^-m E_1[m] (m < n) to scratch; launch 2, a radix-2
 // tile of the pair whose difference output is skipped (POST bound 1), writes
 // y[m] = P_0[m] + P_1[m].
+static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L);
+static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k);
+
 static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id src, buf_id dst)
 {
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
@@ -1078,6 +1084,7 @@ static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id sr
     if (!Mh) {
         return false;
     }
+    const sfh_fixed_kernel *pair = find_fixed(p, SFH_FX_BLUE_PAIR, Mh);
     const uint64_t M = 2 * Mh;
     std::vector<cxl> chirp(n), bvec(M, cxl{0.0L, 0.0L});
     for (uint64_t j = 0; j < n; ++j) {
@@ -1110,8 +1117,47 @@ static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id sr
     const int64_t pre_off = table(p, "hpre" + nkey, pre);
     const int64_t post_off = table(p, "hpost" + nkey, post);
     const int64_t spec_off = table(p, "spec" + nkey, spec);
-    const int64_t one_off = table(p, "one", std::vector<cxl>{cxl{1.0L, 0.0L}});
     p.scratch_elems = std::max(p.scratch_elems, outer * 2 * n);
+    if (pair) {
+        // one launch: the second half of each pair to finish adds the two partial results
+        geometry g;
+        g.L = static_cast<uint32_t>(Mh);
+        g.W = 1;
+        g.wcount = outer;
+        g.unit_dim = SFH_UNIT_COL;
+        g.nblo = 2; // the half h
+        g.in_w = static_cast<int64_t>(n);
+        g.in_j = 1;
+        g.in_blo = 0; // both halves read the same transform
+        g.out_w = static_cast<int64_t>(n);
+        g.out_k = 1;
+        g.out_blo = 0; // and write the same output
+        g.col_div = 1;
+        g.col_mod = 1;
+        g.col_bmul = 1; // col = h
+        launch l = make_launch(p, SFH_MI_BLUE_FUSED, g, src, dst, true, "bluestein-pair");
+        sfh_tile_args &a = l.args;
+        a.pre_jmul = 2;
+        a.pre_cmul = 1;
+        a.nvalid_in = static_cast<uint32_t>(2 * n);
+        a.mid_kmul = 2;
+        a.mid_cmul = 1;
+        a.post_kmul = 2;
+        a.post_cmul = 1;
+        a.nvalid_out = static_cast<uint32_t>(2 * n);
+        a.pair_stride = static_cast<uint32_t>(n);
+        l.conj_post = true;
+        l.pre_off = pre_off;
+        l.mid_off = spec_off;
+        l.post_off = post_off;
+        l.fixed = pair;
+        l.fx_tw_off = fixed_tables(p, *pair);
+        l.desc += " fixed=" + std::to_string(pair->nt) + "x" + std::to_string(pair->L) + " pair";
+        p.pair_transforms = std::max(p.pair_transforms, outer);
+        p.launches.push_back(l);
+        return true;
+    }
+    const int64_t one_off = table(p, "one", std::vector<cxl>{cxl{1.0L, 0.0L}});
     {
         geometry g;
         g.L = static_cast<uint32_t>(Mh);
@@ -1173,12 +1219,32 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
     // chirp c[j] = exp(-i pi j^2/n); b[m] = conj(c[m]) for |m| < n wrapped mod M;
     // X[k] = c[k] * IDFT_M(DFT_M(x c) * DFT_M(b))[k]  (src/sycl/sycl.cpp:268-339, 546-568)
     if (inner == 1 && !p.tune.split) {
-        // Two workgroups per transform, on request (SYCLFFT_HIP_BLUE_HALVES=1).  Measured on
-        // n = 10007: one workgroup's program of two FFTs is latency-bound, so halving its
-        // length only shortens it from ~55 to ~46 us, and the pair-sum launch costs more
-        // than that (batch 128: 63 us against 55 us fused); off by default.
-        const char *e = std::getenv("SYCLFFT_HIP_BLUE_HALVES");
-        if (e && std::atoi(e) != 0 && plan_bluestein_halves(p, outer, n, src, dst)) {
+        // Two workgroups per transform when one per transform would leave at least half of
+        // the CUs idle and the length-specialized pair kernel exists (its second workgroup
+        // to finish adds the halves in the same launch).  Measured on n = 10007 (fused
+        // 20160-point kernel against the pair of 10080-point halves): batch 64 44 -> 32 us,
+        // batch 128 44 -> 36 us, batch 256 50 -> 65 us (two rounds of pairs).  The generic
+        // form of the halves needs a second launch for the sum and never pays (batch 128:
+        // 63 against 55 us), so it runs only on request.  SYCLFFT_HIP_BLUE_HALVES=1 forces
+        // the halves, =0 never takes them.
+        int force = -1;
+        if (const char *e = std::getenv("SYCLFFT_HIP_BLUE_HALVES")) {
+            if (std::strcmp(e, "0") == 0) {
+                force = 0;
+            } else if (std::strcmp(e, "1") == 0) {
+                force = 1;
+            } else {
+                throw sfh_error(SYCLFFT_HIP_STATUS_INVALID_ARGUMENT, "SYCLFFT_HIP_BLUE_HALVES is not 0 or 1");
+            }
+        }
+        bool take = force == 1;
+        if (force < 0) {
+            const uint64_t cus = static_cast<uint64_t>(std::max(0, sfh_device_cus()));
+            const uint64_t cap = static_cast<uint64_t>(max_cap(p));
+            const uint64_t Mh = smooth_at_least(n, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+            take = cus > 0 && 2 * outer <= cus && Mh && find_fixed(p, SFH_FX_BLUE_PAIR, Mh) != nullptr;
+        }
+        if (take && plan_bluestein_halves(p, outer, n, src, dst)) {
             return;
         }
     }
@@ -1525,6 +1591,9 @@ static void release(plan *p)
     if (p->tables) {
         (void)hipFree(p->tables);
     }
+    if (p->pair_counts) {
+        (void)hipFree(p->pair_counts);
+    }
     for (int k = 0; k < plan::k_unit_slots; ++k) {
         if (p->unit_free[k]) {
             (void)hipEventSynchronize(p->unit_free[k]);
@@ -1575,6 +1644,29 @@ static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k)
     return table(p, key, v);
 }
 
+// A length-specialized kernel of the plan's precision for `mode_index` and length L (null:
+// none within the LDS limit; none at all with SYCLFFT_HIP_FIXED=0).
+static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L)
+{
+    if (const char *fe = std::getenv("SYCLFFT_HIP_FIXED")) {
+        if (std::strcmp(fe, "0") == 0) {
+            return nullptr;
+        }
+    }
+    if (p.cfg.precision != SYCLFFT_HIP_PRECISION_VALU) {
+        return nullptr;
+    }
+    int nk = 0;
+    const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    for (int i = 0; i < nk; ++i) {
+        if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L
+            && ks[i].lds_bytes <= k_lds_limit) {
+            return &ks[i];
+        }
+    }
+    return nullptr;
+}
+
 // Length-specialized kernels (fft_fixed.hip) for launches that fit their contract: one
 // column per tile, contiguous points, no LDS load/store loops, one tile per workgroup,
 // plain or Bluestein-fused mode, VALU precision.  SYCLFFT_HIP_FIXED=0 keeps every launch on
@@ -1596,7 +1688,7 @@ static void choose_fixed(plan &p)
     const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
     for (auto &l : p.launches) {
         const sfh_tile_args &a = l.args;
-        if (a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0
+        if (l.fixed || a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0
             || l.grid < a.ntiles || (l.mode != SFH_MI_PLAIN && l.mode != SFH_MI_BLUE_FUSED)) {
             continue;
         }
@@ -1635,6 +1727,12 @@ static hipError_t launch_tile(const plan &p, const launch &l, sfh_tile_args &a, 
         }
         a.twtab = l.fx_twtab;
         a.lds_bytes = l.fixed->lds_bytes;
+        if (l.fixed->mode_index == SFH_FX_BLUE_PAIR) {
+            if (!a.pair_buf || !p.pair_counts) {
+                return hipErrorInvalidConfiguration;
+            }
+            a.pair_count = p.pair_counts;
+        }
         return static_cast<hipError_t>(sfh_launch_fixed(l.fixed, &a, stream));
     }
     return static_cast<hipError_t>(sfh_launch_tile(&p.variants[l.variant], l.mode, &a, grid, stream));
@@ -1815,6 +1913,11 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     }
     if (p->scratch_elems) {
         hip_check(hipMalloc(&p->scratch, p->scratch_elems * p->esize), "hipMalloc(scratch)");
+    }
+    if (p->pair_transforms) {
+        hip_check(hipMalloc(reinterpret_cast<void **>(&p->pair_counts), p->pair_transforms * sizeof(uint32_t)),
+                  "hipMalloc(pair counters)");
+        hip_check(hipMemset(p->pair_counts, 0, p->pair_transforms * sizeof(uint32_t)), "hipMemset(pair counters)");
     }
     // The serialization event only orders device work between streams and gates plan
     // destruction, so it needs no system-scope fence: recording it must not write back
@@ -2209,6 +2312,7 @@ static void execute_checked(
     void *bufs[3] = {const_cast<void *>(in), out, scratch};
     for (auto &l : p.launches) {
         sfh_tile_args a = l.args;
+        a.pair_buf = scratch;
         a.in = static_cast<const unsigned char *>(bufs[l.src]) + l.src_off * static_cast<int64_t>(p.esize);
         a.out = static_cast<unsigned char *>(bufs[l.dst]) + l.dst_off * static_cast<int64_t>(p.esize);
         hip_check(launch_tile(p, l, a, l.grid, stream), "tile kernel launch");
@@ -2399,6 +2503,7 @@ static void execute_units(
         const void *user[2] = {ins[0], outs[0]}; // BUF_IN, BUF_OUT bases of the affine case
         a.in = (l.src == BUF_SCRATCH) ? p.scratch : (affine ? user[l.src] : nullptr);
         a.out = (l.dst == BUF_SCRATCH) ? p.scratch : (affine ? const_cast<void *>(user[l.dst]) : nullptr);
+        a.pair_buf = p.scratch;
         a.in_units = tabs[i].in;
         a.out_units = tabs[i].out;
         const uint64_t nbhi = a.ntiles / (a.nwt * a.nblo);

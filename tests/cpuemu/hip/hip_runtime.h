@@ -4,6 +4,7 @@
 
 #include "hip_runtime_api.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <functional>
@@ -14,6 +15,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define atomicAdd(p, v) emu_atomic_add_u32((p), (v))
+#define __threadfence() std::atomic_thread_fence(std::memory_order_seq_cst)
+
 #define __shared__ static
 
 extern thread_local dim3 threadIdx;
@@ -55,3 +58,21 @@ void emu_register_queue_kernel(const void *fn, void (*call)(void **args));
 
 // tile kernels are launched through hipLaunchKernel with one sfh_tile_args argument
 void emu_register_tile_kernel(const void *fn, void (*call)(const void *fn, void **args));
+
+// agent-scope atomic accesses (the emulation runs workgroups on host threads)
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <typename T>
+inline void __hip_atomic_store(T *p, T v, int, int)
+{
+    __atomic_store_n(p, v, __ATOMIC_SEQ_CST);
+}
+template <typename T>
+inline T __hip_atomic_load(const T *p, int, int)
+{
+    return __atomic_load_n(p, __ATOMIC_SEQ_CST);
+}
+template <typename T>
+inline T __hip_atomic_fetch_add(T *p, T v, int, int)
+{
+    return emu_atomic_add_u32(p, v);
+}

@@ -130,6 +130,33 @@ def test_repeated_execution_serializes():
     assert rel_l2(a, oracle_fftn(x, [n], b)) < tol_l2(n, False)
 
 
+@pytest.mark.parametrize("in_place", [False, True])
+def test_bluestein_pair_repeats(in_place, monkeypatch):
+    """The Bluestein form over two workgroups per transform (n = 10007, half length 10080):
+    the second workgroup of each pair to finish adds the halves, counting arrivals on a
+    per-transform counter that every execution advances by two.  Repeated executions must
+    give bit-identical results, within the oracle bound."""
+    monkeypatch.setenv("SYCLFFT_HIP_BLUE_HALVES", "1")
+    n, b = 10007, 48
+    x = rnd(n * b, 17)
+    say(f"case bluestein pair repeats in_place={in_place}")
+    p = sh.Plan([n], b, 1, in_place=in_place, normalization="orthogonal")
+    assert "pair" in p.describe()
+    outs = []
+    for _ in range(3):
+        xd = dev_tensor(x, False)
+        if in_place:
+            p.execute(xd)
+            outs.append(xd.cpu().numpy())
+        else:
+            yd = torch.empty_like(xd)
+            p.execute(xd, yd)
+            outs.append(yd.cpu().numpy())
+    p.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert rel_l2(outs[0], oracle_fftn(x, [n], b, 1, "orthogonal")) < tol_l2(n, False, 4.0)
+
+
 def test_concurrent_plans_and_impulse():
     """Two plans on one stream, impulse inputs (tests/sycl_tests.cpp:410-435)."""
     say("case concurrent plans")
