@@ -52,8 +52,12 @@ const sfh_fixed_kernel *table(int *count)
         // split tiles, two workgroups per CU
         entry<double, 768, 6, 0, false, true, false, radices<9, 9, 9, 9>, none>(),
         // complex<float> powers of two, pad layout where linear, split tiles
-        entry<float, 512, 4, 0, true, true, false, radices<8, 16, 8, 8>, none>(),
+        // 8192: the radix-2 pass first keeps every pass boundary pad-linear (no bank conflicts)
+        entry<float, 512, 4, 0, true, true, false, radices<2, 16, 16, 16>, none>(),
         entry<float, 1024, 8, 0, true, true, false, radices<16, 16, 16, 4>, none>(),
+        // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
+        // the 32 points of each work-item stay in registers across every exchange
+        entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 16, 8>, none>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

@@ -861,6 +861,51 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
     return l;
 }
 
+static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L);
+static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k);
+
+// A launch that only a length-specialized kernel runs (no generic variant holds its tile):
+// one contiguous transform of k.L points per workgroup, columns `g.wcount`.
+static launch make_fixed_launch(plan &p, const sfh_fixed_kernel &k, const geometry &g, buf_id src, buf_id dst,
+                                const std::string &what)
+{
+    launch l;
+    l.mode = SFH_MI_PLAIN;
+    l.src = src;
+    l.dst = dst;
+    l.variant = 0; // unused: the fixed kernel runs every execution of this launch
+    sfh_tile_args &a = l.args;
+    a.L = static_cast<uint32_t>(g.L);
+    a.LP = a.L;
+    a.W = 1;
+    a.logW = 0;
+    a.tpc = static_cast<uint32_t>(k.nt);
+    a.wcount = g.wcount;
+    a.nwt = g.wcount;
+    a.nblo = 1;
+    a.ntiles = g.wcount;
+    a.in_w = static_cast<int32_t>(g.in_w);
+    a.in_j = 1;
+    a.out_w = static_cast<int32_t>(g.out_w);
+    a.out_k = 1;
+    a.col_div = make_fastdiv(1);
+    a.col_mod = make_fastdiv(1);
+    a.out_sx = 1.0;
+    a.out_sy = 1.0;
+    a.unit_dim = SFH_UNIT_COL;
+    l.unit_dim = SFH_UNIT_COL;
+    l.unit_extent = g.wcount;
+    l.unit_in_stride = g.in_w;
+    l.unit_out_stride = g.out_w;
+    l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
+    l.fixed = &k;
+    l.fx_tw_off = fixed_tables(p, k);
+    a.lds_bytes = k.lds_bytes;
+    l.desc = what + " L=" + std::to_string(g.L) + " W=1 tiles=" + std::to_string(a.ntiles) + " fixed="
+             + std::to_string(k.nt) + "x" + std::to_string(k.L);
+    return l;
+}
+
 // Columns per tile for an axis whose columns are `contig` (adjacent in memory) or whose
 // points are contiguous.
 static uint32_t choose_w(const plan &p, uint64_t L, uint64_t wcount, bool contig_cols)
@@ -952,6 +997,18 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
         return;
     }
     if (is_smooth7(n)) {
+        // ---------------- one launch of a length-specialized kernel beyond the generic tiles
+        if (inner == 1 && !p.tune.split && n > static_cast<uint64_t>(max_cap(p)) && n <= 0xffffffffu) {
+            if (const sfh_fixed_kernel *k = find_fixed(p, SFH_MI_PLAIN, n)) {
+                geometry g;
+                g.L = static_cast<uint32_t>(n);
+                g.W = 1;
+                g.wcount = outer;
+                g.in_w = g.out_w = static_cast<int64_t>(n);
+                p.launches.push_back(make_fixed_launch(p, *k, g, src, dst, "single"));
+                return;
+            }
+        }
         // ---------------- single launch ----------------
         {
             geometry g;
@@ -1074,9 +1131,6 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
 // writes P_h[m] = c[m] E_0[m] and c[m] W_M^-m E_1[m] (m < n) to scratch; launch 2, a radix-2
 // tile of the pair whose difference output is skipped (POST bound 1), writes
 // y[m] = P_0[m] + P_1[m].
-static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L);
-static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k);
-
 static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id src, buf_id dst)
 {
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
@@ -1839,7 +1893,7 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
         const bool allow = pe ? std::atoi(pe) != 0 : p->tune.prefetch != 0;
         const int cus = sfh_device_cus();
         for (auto &l : p->launches) {
-            if (!allow || cus <= 0 || (l.args.flags & SFH_LOAD_LDS)) {
+            if (!allow || cus <= 0 || (l.args.flags & SFH_LOAD_LDS) || l.fixed) {
                 continue;
             }
             const int pf_mode = l.mode == SFH_MI_PLAIN ? SFH_MI_PLAIN_PF : (l.mode == SFH_MI_T4 ? SFH_MI_T4_PF : -1);
@@ -1873,6 +1927,9 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
         }
         const int cus = sfh_device_cus();
         for (auto &l : p->launches) {
+            if (l.fixed) {
+                continue;
+            }
             const int twin = force == 0 ? -1 : global_table_twin(*p, l.variant);
             if (twin < 0) {
                 continue;

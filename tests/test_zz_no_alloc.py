@@ -46,7 +46,7 @@ def device_input(n, double, seed):
 # smaller one that follows the first in the same process
 CASES = [
     ([1 << 18], 8, False, -1, "none", False, None),        # four-step, 16 MiB of scratch
-    ([32768], 4, False, 1, "backward", False, None),       # four-step, smaller
+    ([1 << 16], 4, False, 1, "backward", False, None),     # four-step, smaller
     ([10007], 4, False, -1, "orthogonal", False, "s1w0p0g0"),  # three-launch Bluestein
     ([16384], 8, True, 1, "none", True, None),            # f64 four-step, in place
     ([64, 4096], 2, False, -1, "forward", False, None),    # 2-D with a four-step-free plan
