@@ -11,7 +11,7 @@ namespace
 using sfh::fx::radices;
 
 template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2>
-sfh_fixed_kernel entry()
+sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
 {
     using prog = sfh::fx::program<R1, R2, MODE, PAD>;
     sfh_fixed_kernel k{};
@@ -22,6 +22,7 @@ sfh_fixed_kernel entry()
     k.L = static_cast<uint32_t>(prog::L);
     // LDS bytes per tile: one complex value per slot, or one real with the split exchange
     k.lds_bytes = static_cast<uint32_t>(prog::slots() * (SPLIT ? sizeof(T) : 2 * sizeof(T)));
+    k.max_tiles_per_cu = max_tiles_per_cu;
     k.n1 = R1::n;
     for (int i = 0; i < R1::n; ++i) {
         k.r1[i] = R1::v[i];
@@ -52,6 +53,9 @@ const sfh_fixed_kernel *table(int *count)
         // split tiles, two workgroups per CU
         entry<double, 768, 6, 0, false, true, false, radices<9, 9, 9, 9>, none>(),
         // complex<float> powers of two, pad layout where linear, split tiles
+        // complex<float> 4096 for batches of at most one transform per CU (latency-bound single
+        // round): 512 work-items per transform, every pass boundary pad-linear
+        entry<float, 512, 2, 0, true, false, false, radices<4, 8, 16, 8>, none>(1),
         // 8192: the radix-2 pass first keeps every pass boundary pad-linear (no bank conflicts)
         entry<float, 512, 4, 0, true, true, false, radices<2, 16, 16, 16>, none>(),
         entry<float, 1024, 8, 0, true, true, false, radices<16, 16, 16, 4>, none>(),

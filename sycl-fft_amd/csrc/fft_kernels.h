@@ -50,6 +50,9 @@ struct sfh_fixed_kernel
     int nt;
     uint32_t L;
     uint32_t lds_bytes;
+    // used only for launches of at most this many tiles per CU (0: any): entries that spread
+    // one transform over more work-items, for batches that give every CU one tile or less
+    int max_tiles_per_cu;
     int n1, n2;
     int r1[8], r2[8];
 };

@@ -1714,7 +1714,7 @@ static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_
     const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
     for (int i = 0; i < nk; ++i) {
         if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L
-            && ks[i].lds_bytes <= k_lds_limit) {
+            && ks[i].lds_bytes <= k_lds_limit && ks[i].max_tiles_per_cu == 0) {
             return &ks[i];
         }
     }
@@ -1740,6 +1740,7 @@ static void choose_fixed(plan &p)
     }
     int nk = 0;
     const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    const uint64_t cus = static_cast<uint64_t>(std::max(0, sfh_device_cus()));
     for (auto &l : p.launches) {
         const sfh_tile_args &a = l.args;
         if (l.fixed || a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0
@@ -1748,8 +1749,9 @@ static void choose_fixed(plan &p)
         }
         for (int i = 0; i < nk; ++i) {
             const sfh_fixed_kernel &k = ks[i];
-            if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L
-                && k.lds_bytes <= k_lds_limit) {
+            if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L && k.lds_bytes <= k_lds_limit
+                && (k.max_tiles_per_cu == 0
+                    || (cus > 0 && a.ntiles <= cus * static_cast<uint64_t>(k.max_tiles_per_cu)))) {
                 l.fixed = &k;
                 l.fx_tw_off = fixed_tables(p, k);
                 l.desc += " fixed=" + std::to_string(k.nt) + "x" + std::to_string(k.L);
