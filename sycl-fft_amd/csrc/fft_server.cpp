@@ -475,6 +475,24 @@ private:
             const uint64_t es = head->r.scalar == SYCLFFT_HIP_SCALAR_DOUBLE ? 16 : 8;
             plan_bytes = (2 * capacity + 4) * elems_of(head->r) * es;
         }
+        // The resident plan chosen for the group is pinned while room is made, so the
+        // eviction cannot destroy it (the group would run on a freed plan).
+        struct pin_guard
+        {
+            plan_entry *e;
+            explicit pin_guard(plan_entry *x) : e(x)
+            {
+                if (e) {
+                    ++e->inflight;
+                }
+            }
+            ~pin_guard()
+            {
+                if (e) {
+                    --e->inflight;
+                }
+            }
+        } pin(pe);
         if (!make_room(bytes + plan_bytes)) {
             if (!inflight_.empty()) {
                 return false; // wait for completions to free memory

@@ -134,7 +134,7 @@ int main()
     };
     std::mt19937_64 rng(20260005);
     std::uniform_real_distribution<double> u(-1.0, 1.0);
-    const std::vector<std::vector<uint64_t>> shapes = {{64}, {300}, {4096}, {1009}, {13, 16}, {8, 8, 4}};
+    const std::vector<std::vector<uint64_t>> shapes = {{64}, {300}, {4096}, {1009}, {13, 16}, {8, 8, 4}, {32768}, {10007}, {2520}};
     std::vector<req_case> cases;
     for (int k = 0; k < 36; ++k) {
         req_case c;
@@ -256,6 +256,62 @@ int main()
         syclfft_hip_server_destroy(small);
         hipFree(big.din);
         hipFree(big.dout);
+    }
+    {
+        // Eviction under a tight budget: three 1024-point plans (different keys) fill it, then
+        // a request of the first, least recently used shape arrives.  Its resident plan is the
+        // one to use, and it must survive the eviction that makes room for the request's
+        // buffers (another plan goes, or the request waits).
+        std::vector<req_case> ev;
+        const int dirs[3] = {-1, 1, -1};
+        const uint32_t norms[3] = {0, 0, 2};
+        for (int k = 0; k < 3; ++k) {
+            req_case c;
+            c.lens = {1024};
+            c.batch = k == 0 ? 2 : 1; // the first shape's requests need the most room
+            c.dbl = false;
+            c.dir = dirs[k];
+            c.norm = norms[k];
+            c.in_place = false;
+            c.x.resize(2 * 1024 * c.batch);
+            for (auto &v : c.x) {
+                v = std::uniform_real_distribution<double>(-1, 1)(rng);
+            }
+            upload(c);
+            ev.push_back(std::move(c));
+        }
+        // plan charge (2 * 8 + 4) * 1024 * 8 B each: three plans and one 16 KiB request fit,
+        // three plans and the first shape's 32 KiB request do not
+        void *tight = syclfft_hip_server_create(-1, 3 * 163840 + 16384 + 4096, 1, 8, err, sizeof(err));
+        check(tight != nullptr, "tight server_create", err);
+        const int order[6] = {0, 1, 2, 0, 1, 2};
+        for (int k = 0; k < 6; ++k) {
+            req_case &c = ev[order[k]];
+            const syclfft_hip_request_v1 r = make_req(c, 100 + k);
+            uint64_t t = 0;
+            int st = syclfft_hip_server_submit(tight, &r, &t, err, sizeof(err));
+            syclfft_hip_completion_v1 comp{};
+            if (st == SYCLFFT_HIP_STATUS_OK) {
+                st = syclfft_hip_server_wait(tight, t, &comp, err, sizeof(err));
+            }
+            check(st == SYCLFFT_HIP_STATUS_OK, "tight budget request " + std::to_string(k), err);
+            const uint64_t total = 1024 * c.batch;
+            const std::vector<double> got = download(c);
+            std::vector<double> want(2 * total);
+            const uint64_t lens[3] = {1024, 1, 1};
+            oracle_fftn_ld(1, lens, c.batch, c.x.data(), want.data(), c.dir, static_cast<int>(c.norm));
+            const double e = rel_l2(got, want);
+            check(e < tol_of(c), "tight budget result " + std::to_string(k));
+        }
+        syclfft_hip_server_stats_v1 s{};
+        syclfft_hip_server_stats(tight, &s);
+        check(s.plans_evicted > 0, "plans evicted under the tight budget");
+        check(s.peak_footprint_bytes <= s.budget_bytes, "tight footprint within budget");
+        syclfft_hip_server_destroy(tight);
+        for (auto &c : ev) {
+            hipFree(c.din);
+            hipFree(c.dout);
+        }
     }
     syclfft_hip_server_destroy(srv);
     for (auto &c : cases) {
