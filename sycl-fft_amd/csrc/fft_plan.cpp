@@ -279,6 +279,7 @@ struct plan
     std::vector<unsigned char> host_tables;
     std::map<std::string, int64_t> table_cache;
     void *tables{};
+    uint64_t tables_bytes{};
     uint64_t scratch_elems{};
     void *scratch{};
     // Bluestein pair launches: one arrival counter per transform (zeroed at creation)
@@ -1966,6 +1967,7 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     choose_fixed(*p);
     if (!p->host_tables.empty()) {
         hip_check(hipMalloc(&p->tables, p->host_tables.size()), "hipMalloc(tables)");
+        p->tables_bytes = p->host_tables.size();
         hip_check(
             hipMemcpy(p->tables, p->host_tables.data(), p->host_tables.size(), hipMemcpyHostToDevice),
             "hipMemcpy(tables)");
@@ -3568,6 +3570,25 @@ extern "C" __attribute__((visibility("hidden"))) int sfh_plan_execute_with_scrat
         sfh::set_err(err, cap, "unknown error");
         return SYCLFFT_HIP_STATUS_EXECUTION_FAILED;
     }
+}
+
+// Device memory a complex plan holds: tables, scratch, arrival counters and its ring of
+// pointer-array tables (0 for a null handle).
+extern "C" __attribute__((visibility("hidden"))) uint64_t sfh_plan_device_bytes(const void *plan)
+{
+    if (!plan) {
+        return 0;
+    }
+    const auto *p = static_cast<const sfh::plan *>(plan);
+    uint64_t b = p->tables_bytes + p->scratch_elems * p->esize + p->pair_transforms * sizeof(uint32_t)
+                 + static_cast<uint64_t>(sfh::plan::k_unit_slots) * p->unit_bytes;
+    if (p->half) {
+        b += sfh_plan_device_bytes(p->half);
+    }
+    if (p->core) {
+        b += sfh_plan_device_bytes(p->core);
+    }
+    return b;
 }
 
 // Device of a plan (-1 for a null handle).

@@ -45,6 +45,8 @@ extern "C" int sfh_plan_execute_many_checked_buffers(
 // fft_plan.cpp: work-queue class of `count` transforms of a plan (-1: none) and a queue
 // execution of checked buffers (entry k: counts[k] transforms of plans[k])
 extern "C" int sfh_plan_queue_class(const void *plan, uint64_t count);
+// fft_plan.cpp: device memory a plan holds
+extern "C" uint64_t sfh_plan_device_bytes(const void *plan);
 extern "C" int sfh_execute_queue_checked_buffers(
     void *const *plans, const uint64_t *counts, const void *const *ins, void *const *outs, size_t n, void *stream,
     void *done, char *err, size_t cap);
@@ -621,7 +623,9 @@ private:
                 fresh->key = head->key;
                 fresh->plan = h;
                 fresh->capacity = capacity;
-                fresh->bytes = plan_bytes;
+                // admitted on an upper bound (scratch for every transform); charged from here
+                // on with what the plan holds, plus room for its pointer-array tables
+                fresh->bytes = std::min(plan_bytes, sfh_plan_device_bytes(h) + (uint64_t(64) << 10));
                 fresh->inflight = 1;
                 pe = fresh.get();
             } else {
@@ -683,6 +687,7 @@ private:
         lk.lock();
         --launching_;
         if (fresh) {
+            footprint_ -= plan_bytes - fresh->bytes;
             by_key_[fresh->key].push_back(fresh.get());
             plans_.push_back(std::move(fresh));
             ++st_.plans_created;

@@ -258,7 +258,7 @@ int main()
         hipFree(big.dout);
     }
     {
-        // Eviction under a tight budget: three 1024-point plans (different keys) fill it, then
+        // Eviction under a tight budget: three 256-point plans (different keys) fill it, then
         // a request of the first, least recently used shape arrives.  Its resident plan is the
         // one to use, and it must survive the eviction that makes room for the request's
         // buffers (another plan goes, or the request waits).
@@ -267,22 +267,23 @@ int main()
         const uint32_t norms[3] = {0, 0, 2};
         for (int k = 0; k < 3; ++k) {
             req_case c;
-            c.lens = {1024};
+            c.lens = {256};
             c.batch = k == 0 ? 2 : 1; // the first shape's requests need the most room
             c.dbl = false;
             c.dir = dirs[k];
             c.norm = norms[k];
             c.in_place = false;
-            c.x.resize(2 * 1024 * c.batch);
+            c.x.resize(2 * 256 * c.batch);
             for (auto &v : c.x) {
                 v = std::uniform_real_distribution<double>(-1, 1)(rng);
             }
             upload(c);
             ev.push_back(std::move(c));
         }
-        // plan charge (2 * 8 + 4) * 1024 * 8 B each: three plans and one 16 KiB request fit,
-        // three plans and the first shape's 32 KiB request do not
-        void *tight = syclfft_hip_server_create(-1, 3 * 163840 + 16384 + 4096, 1, 8, err, sizeof(err));
+        // A plan is charged min(its admission bound (2 * 8 + 4) * 256 * 8 B = 40960 B, what it
+        // holds + 64 KiB) = 40960 B.  Two plans, a new one and a 4 KiB request fit; three
+        // plans and the first shape's 8 KiB request do not.
+        void *tight = syclfft_hip_server_create(-1, 128000, 1, 8, err, sizeof(err));
         check(tight != nullptr, "tight server_create", err);
         const int order[6] = {0, 1, 2, 0, 1, 2};
         for (int k = 0; k < 6; ++k) {
@@ -295,10 +296,10 @@ int main()
                 st = syclfft_hip_server_wait(tight, t, &comp, err, sizeof(err));
             }
             check(st == SYCLFFT_HIP_STATUS_OK, "tight budget request " + std::to_string(k), err);
-            const uint64_t total = 1024 * c.batch;
+            const uint64_t total = 256 * c.batch;
             const std::vector<double> got = download(c);
             std::vector<double> want(2 * total);
-            const uint64_t lens[3] = {1024, 1, 1};
+            const uint64_t lens[3] = {256, 1, 1};
             oracle_fftn_ld(1, lens, c.batch, c.x.data(), want.data(), c.dir, static_cast<int>(c.norm));
             const double e = rel_l2(got, want);
             check(e < tol_of(c), "tight budget result " + std::to_string(k));
