@@ -268,21 +268,33 @@ def serving_requests(nreq, seed):
     return reqs
 
 
-def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, streams=8):
-    """The serving loop under three loads over the same request stream: all requests at
-    once (batched capacity, median of three replays), one at a time (isolated latency), and
-    Poisson arrivals at 80% of the batched request rate (latency under load, the median of
-    three arrival streams by p99).  Request buffers are device-resident:
-    inputs read from one seeded buffer, each request's output in its own region of one
-    arena.  Arrivals are replayed from native threads (syclfft_hip_server_replay)."""
+def serving_bench(sh, torch, dev, nreq=20000, seed=20260005, budget_gb=64, streams=4, capacity_runs=5,
+                  arena_gb=48, load_fracs=(0.5, 0.65)):
+    """The serving loop over one seeded request stream under three kinds of load: all
+    requests at once (batched capacity: the median of `capacity_runs` replays), one at a
+    time (isolated latency), and Poisson arrivals at 80% of the batched request rate
+    (latency under load: the median of three arrival streams by p99), plus one Poisson
+    stream at each of `load_fracs` of that rate (the latency-load curve).  Request buffers are
+    device-resident: inputs read from one seeded buffer; outputs in an arena of `arena_gb` GiB
+    that successive requests fill in turn and wrap around (requests that far apart are never
+    in flight together under the 64 GiB admission budget; the outputs are not checked here).
+    Arrivals are replayed from native threads (syclfft_hip_server_replay).  `streams` stays at
+    the box's hardware-queue count, so no two streams share a queue."""
     reqs = serving_requests(nreq, seed)
     flops_total = sum(flops(r["n"], r["batch"]) for r in reqs)
     src = torch.empty((1 << 22) * 2, dtype=torch.float64, device=dev).uniform_(-1, 1)
     src64 = torch.view_as_complex(src.view(-1, 2))
     src32 = torch.view_as_complex(src[: (1 << 23)].float().view(-1, 2))
     sizes = [r["n"] * r["batch"] * (16 if r["double"] else 8) for r in reqs]
-    offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in sizes])])
-    arena = torch.empty(int(offs[-1]), dtype=torch.uint8, device=dev)
+    ring = min(int(arena_gb) << 30, int(sum((b + 255) // 256 * 256 for b in sizes)))
+    offs, at = [], 0
+    for b in sizes:
+        b = (b + 255) // 256 * 256
+        if at + b > ring:
+            at = 0
+        offs.append(at)
+        at += b
+    arena = torch.empty(ring, dtype=torch.uint8, device=dev)
     base = arena.data_ptr()
     torch.cuda.synchronize()
     srv = sh.Server(budget_bytes=budget_gb << 30, streams=streams, max_batch=512)
@@ -293,27 +305,38 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
     def span(recs):
         return max(c["done_at"] for c in recs) - min(c["submit_at"] for c in recs)
 
+    def poisson(frac, s):
+        g = np.random.default_rng(s)
+        return np.cumsum(g.exponential(1.0 / (frac * rate), len(reqs)))
+
     srv.replay(requests)  # warm-up: plans for every shape become resident
     # The all-at-once capacity depends on how the replay threads' submissions happen to
-    # merge into launches, so it is the median of three replays (spread reported).
-    runs = sorted((srv.replay(requests) for _ in range(3)), key=span)
-    batched = runs[1]
+    # merge into launches, so it is the median of several replays (all reported).
+    runs = sorted((srv.replay(requests) for _ in range(capacity_runs)), key=span)
+    batched = runs[len(runs) // 2]
     span_b = span(batched)
     rate = len(reqs) / span_b
     isolated = srv.replay(requests, sequential=True)
     lat_i = np.array([c["latency_s"] for c in isolated]) * 1e3
-    # Poisson arrivals at 80% of the capacity, three arrival streams (seeds seed+1..+3): a
-    # loaded replay of 1500 requests lasts ~40 ms, so its p99 (the 15th-slowest request)
-    # varies between streams; the run with the median p99 is reported, with all three.
+    p99_i = float(np.percentile(lat_i, 99))
+    # Poisson arrivals at 80% of the capacity, three arrival streams (seeds seed+1..+3); the
+    # run with the median p99 is reported, with all three.
     loads = []
     for k in range(3):
-        g = np.random.default_rng(seed + 1 + k)
-        arrivals = np.cumsum(g.exponential(1.0 / (0.8 * rate), len(reqs)))
-        recs = srv.replay(requests, arrivals=arrivals)
+        recs = srv.replay(requests, arrivals=poisson(0.8, seed + 1 + k))
         loads.append((float(np.percentile([c["latency_s"] for c in recs], 99)), recs))
     loads.sort(key=lambda t: t[0])
     loaded = loads[1][1]
     span_l = span(loaded)
+    curve = {}
+    for frac in load_fracs:
+        recs = srv.replay(requests, arrivals=poisson(frac, seed + 11))
+        lat = np.array([c["latency_s"] for c in recs]) * 1e3
+        curve[str(frac)] = {"offered_request_rate": round(frac * rate, 1),
+                            "delivered_frac_of_batched": round(span_b / span(recs), 4),
+                            "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                            "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                            "p99_over_isolated_p99": round(float(np.percentile(lat, 99)) / p99_i, 3)}
     st = srv.stats()
     srv.close()
     del arena, src, src64, src32
@@ -325,7 +348,7 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
         "sample": f"{len(reqs)} requests, seed {seed}: lengths 50% pow2 2^8..2^20, 30% 7-smooth 240..500000 "
                   f"(24 sizes), 20% primes 257..65537 (12 sizes), 10% 2-D; batch log-uniform 1..512 capped at "
                   f"2^22 points; 80% f32 / 20% f64; random direction and normalization; budget {budget_gb} GiB; "
-                  f"{streams} streams; arrivals replayed natively",
+                  f"{streams} streams; outputs in a {ring >> 30} GiB ring; arrivals replayed natively",
         "batched_gflops": round(flops_total / span_b / 1e9, 2),
         "batched_request_rate": round(rate, 1),
         "batched_request_rate_runs": [round(len(reqs) / span(r), 1) for r in runs],
@@ -336,12 +359,13 @@ def serving_bench(sh, torch, dev, nreq=1500, seed=20260005, budget_gb=64, stream
         "p99_ms": pct(lat_l, 99),
         "isolated_p50_ms": pct(lat_i, 50),
         "isolated_p99_ms": pct(lat_i, 99),
-        "p99_over_isolated_p99": round(float(np.percentile(lat_l, 99) / np.percentile(lat_i, 99)), 3),
-        "p99_over_isolated_p99_runs": [round(t[0] * 1e3 / float(np.percentile(lat_i, 99)), 3) for t in loads],
+        "p99_over_isolated_p99": round(float(np.percentile(lat_l, 99) / p99_i), 3),
+        "p99_over_isolated_p99_runs": [round(t[0] * 1e3 / p99_i, 3) for t in loads],
         "per_request_slowdown_p50": pct(slow, 50),
         "per_request_slowdown_p99": pct(slow, 99),
         "mean_group_size_batched": round(float(np.mean([c["group_size"] for c in batched])), 2),
         "mean_group_size_loaded": round(float(np.mean([c["group_size"] for c in loaded])), 2),
+        "load_curve": curve,
         "launches": st["launches"],
         "plans_resident": st["plans_resident"],
         "peak_footprint_gib": round(st["peak_footprint_bytes"] / (1 << 30), 3),
@@ -566,7 +590,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-overlap", action="store_true", help="skip the three-stream report")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
-    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--no-precision", action="store_true", help="skip the split-precision MFMA comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

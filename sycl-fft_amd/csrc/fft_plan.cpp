@@ -1888,6 +1888,8 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
     }
 
     device_guard guard(dev);
+    // length-specialized kernels first: the generic-only forms below skip their launches
+    choose_fixed(*p);
     // Persistent prefetching form for plain and four-step-A launches with more tiles than
     // resident workgroups: SYCLFFT_HIP_PREFETCH=1 enables it (measured slower than one tile
     // per workgroup at full occupancy on the batched shapes, so off by default).
@@ -1964,7 +1966,6 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
             l.desc += " tw=global lds=" + std::to_string(data_bytes);
         }
     }
-    choose_fixed(*p);
     if (!p->host_tables.empty()) {
         hip_check(hipMalloc(&p->tables, p->host_tables.size()), "hipMalloc(tables)");
         p->tables_bytes = p->host_tables.size();
