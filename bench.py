@@ -762,6 +762,7 @@ def main():
     # as stale (null) instead.
     traffic, traffic_note = None, "no profile row"
     dom_plan = dom["fwd"].describe().strip().replace("\n", " | ")
+    launches_of = lambda d: d.split(" | tuning ")[0].strip()  # noqa: E731
     tfile = os.path.join(ROOT, "profiles", "traffic_c2_roundtrip.txt")
     if os.path.exists(tfile):
         with open(tfile) as f:
@@ -770,7 +771,7 @@ def main():
                     continue
                 head, _, plan_txt = row.partition(" plan=")
                 tb = [int(fld.split("=", 1)[1]) for fld in head.split() if fld.startswith("traffic_bytes=")]
-                if tb and plan_txt.strip() == dom_plan:
+                if tb and launches_of(plan_txt) == launches_of(dom_plan):
                     traffic, traffic_note = tb[0], "profiled plan matches"
                 elif tb:
                     traffic_note = "stale: profiled plan differs from this run's plan"

@@ -2158,7 +2158,10 @@ static tuning autotune(const syclfft_hip_plan_config_v1 *cfg, int dev)
         } else if (q->split_launches < base_split) {
             return -1.0;
         }
-        const std::string d = describe_plan(*q);
+        // keyed by the launches alone (not the tuning line), so a tuning that yields the same
+        // launches as one already timed is not timed again and cannot win on noise
+        std::string d = describe_plan(*q);
+        d = d.substr(0, d.rfind("tuning "));
         const auto it = seen.find(d);
         if (it != seen.end()) {
             return it->second;

@@ -4,7 +4,9 @@ four instead of three; 4097..5120 points: two instead of one) run the twin once 
 tiles outnumber the LDS-table variant's resident slots.  Results are checked
 against the FP64 oracle (relative L2 <= 2 log2(N) u) and against the same plan built with
 the twin switched off (SYCLFFT_HIP_TW_GLOBAL=0): the twin reads the same table values and
-runs the same arithmetic, so the outputs are bit-identical.
+runs the same arithmetic, so the outputs are bit-identical.  The length-specialized kernels
+are switched off here (SYCLFFT_HIP_FIXED=0): they take some of these lengths (2520) first,
+and the twin is a form of the generic kernel.
 """
 import os
 import re
@@ -30,10 +32,12 @@ def say(msg):
 
 def run(n, batch, direction, norm, x, twin_env):
     old = os.environ.get("SYCLFFT_HIP_TW_GLOBAL")
+    old_fixed = os.environ.get("SYCLFFT_HIP_FIXED")
     if twin_env is None:
         os.environ.pop("SYCLFFT_HIP_TW_GLOBAL", None)
     else:
         os.environ["SYCLFFT_HIP_TW_GLOBAL"] = twin_env
+    os.environ["SYCLFFT_HIP_FIXED"] = "0"
     try:
         p = sh.Plan([n], batch, direction, double=True, normalization=norm)
     finally:
@@ -41,6 +45,10 @@ def run(n, batch, direction, norm, x, twin_env):
             os.environ.pop("SYCLFFT_HIP_TW_GLOBAL", None)
         else:
             os.environ["SYCLFFT_HIP_TW_GLOBAL"] = old
+        if old_fixed is None:
+            os.environ.pop("SYCLFFT_HIP_FIXED", None)
+        else:
+            os.environ["SYCLFFT_HIP_FIXED"] = old_fixed
     desc = p.describe().strip()
     xd = torch.from_numpy(x).to("cuda")
     yd = torch.empty_like(xd)
