@@ -182,14 +182,19 @@ def host_fftw_baseline(sample_scale=16, target_s=8.0):
 def cache_bench(sh, torch, dev, reps=10):
     """Working sets under 256 MiB, warm (back-to-back executions, data resident in the
     caches) and cold (a 1 GiB write between executions evicts L2 and the MALL): the plain
-    fp32 N=4096 batch 256 forward and the Bluestein f32 N=10007 batch 128 forward."""
+    fp32 N=4096 batch 256 forward, and the forward launches of the round trip's three
+    sub-workloads (f64 2520x1024, f64 6561x512, Bluestein f32 10007x128)."""
     out = {}
     stream = torch.cuda.current_stream(dev)
     scrub = torch.empty(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
-    for name, n, b, norm in (("c1_f32_4096x256", 4096, 256, "none"), ("c2_f32_10007x128", 10007, 128, "orthogonal")):
-        x = torch.view_as_complex((torch.rand((n * b, 2), device=dev) * 2 - 1).contiguous())
+    for name, n, b, norm, double in (("c1_f32_4096x256", 4096, 256, "none", False),
+                                     ("c2_f64_2520x1024", 2520, 1024, "orthogonal", True),
+                                     ("c2_f64_6561x512", 6561, 512, "orthogonal", True),
+                                     ("c2_f32_10007x128", 10007, 128, "orthogonal", False)):
+        rdt = torch.float64 if double else torch.float32
+        x = torch.view_as_complex((torch.rand((n * b, 2), device=dev, dtype=rdt) * 2 - 1).contiguous())
         y = torch.empty_like(x)
-        p = sh.Plan([n], b, sh.FORWARD, normalization=norm, measure=True)
+        p = sh.Plan([n], b, sh.FORWARD, double=double, normalization=norm, measure=True)
         for _ in range(3):
             p.execute(x, y)
         a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -208,7 +213,7 @@ def cache_bench(sh, torch, dev, reps=10):
             b_.record(stream)
             torch.cuda.synchronize()
             cold.append(a.elapsed_time(b_))
-        byts = 2 * n * b * 8
+        byts = 2 * n * b * (16 if double else 8)
         out[name] = {"warm_ms": round(warm, 4), "cold_ms": round(float(np.median(cold)), 4),
                      "warm_gbs": round(byts / (warm * 1e-3) / 1e9, 1),
                      "cold_gbs": round(byts / (float(np.median(cold)) * 1e-3) / 1e9, 1)}

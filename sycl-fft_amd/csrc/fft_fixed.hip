@@ -52,16 +52,22 @@ const sfh_fixed_kernel *table(int *count)
         // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB
         // split tiles, two workgroups per CU
         entry<double, 768, 6, 0, false, true, false, radices<9, 9, 9, 9>, none>(),
-        // complex<float> powers of two, pad layout where linear, split tiles
-        // complex<float> 4096 for batches of at most one transform per CU (latency-bound single
-        // round): 512 work-items per transform, every pass boundary pad-linear
+        // complex<float> powers of two, pad layout where linear.  Measured at 512 MiB batches
+        // against 2-4x the work-items per transform: fewer work-items with more points each
+        // (more independent butterflies per wave between barriers) ran 2-5% faster at every
+        // length; the register file then bounds the tiles per CU (about 256 KiB of points).
+        // 4096 for batches of at most one transform per CU (latency-bound single round): 512
+        // work-items per transform, every pass boundary pad-linear
         entry<float, 512, 2, 0, true, false, false, radices<4, 8, 16, 8>, none>(1),
-        // 8192: the radix-2 pass first keeps every pass boundary pad-linear (no bank conflicts)
-        entry<float, 512, 4, 0, true, true, false, radices<2, 16, 16, 16>, none>(),
-        entry<float, 1024, 8, 0, true, true, false, radices<16, 16, 16, 4>, none>(),
+        // 4096, larger batches: 128 work-items x 32 points, split tiles of 16.5 KiB
+        entry<float, 128, 4, 0, true, true, false, radices<4, 8, 16, 8>, none>(),
+        // 8192: the radix-2 pass between two radix-16 passes keeps every pass boundary
+        // pad-linear (no bank conflicts)
+        entry<float, 256, 2, 0, true, true, false, radices<16, 2, 16, 16>, none>(),
+        entry<float, 256, 2, 0, true, true, false, radices<16, 16, 16, 4>, none>(),
         // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
-        // the 32 points of each work-item stay in registers across every exchange
-        entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 16, 8>, none>(),
+        // the 64 points of each work-item stay in registers across every exchange
+        entry<float, 512, 2, 0, true, true, false, radices<16, 16, 16, 8>, none>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

@@ -127,40 +127,47 @@ def test_bench_plan_full_batch(case):
 
 
 # Forced autotuner forms on the bench's shapes.  `expect` names the describe() marker of
-# forms that change the launch itself where the shape admits them.
+# forms that change the launch itself where the shape admits them.  The length-specialized
+# kernels (4096 ... 32768, f64 2520 and 6561, the Bluestein convolution of 10007) have no
+# persistent or column-width forms: there those forms leave the launch as it is, and the
+# last shape (the generic kernel forced with SYCLFFT_HIP_FIXED=0) is where the persistent
+# form runs.
 FORMS = ["s1w0p0g0", "s0w-1p0g0", "s0w1p0g0", "s0w0p1g0", "s0w0p0g1"]
 FORM_SHAPES = [
-    (4096, 256, False, "none"), (2520, 1024, True, "orthogonal"), (6561, 512, True, "orthogonal"),
-    (10007, 128, False, "orthogonal"), (16384, 4096, False, "none"), (1 << 20, 16, False, "none"),
-    (4096, SWEEP_POINTS >> 12, False, "none"), (8192, SWEEP_POINTS >> 13, False, "none"),
-    (1 << 15, SWEEP_POINTS >> 15, False, "none"), (1 << 18, SWEEP_POINTS >> 18, False, "none"),
+    (4096, 256, False, "none", True), (2520, 1024, True, "orthogonal", True),
+    (6561, 512, True, "orthogonal", True), (10007, 128, False, "orthogonal", True),
+    (16384, 4096, False, "none", True), (1 << 20, 16, False, "none", True),
+    (4096, SWEEP_POINTS >> 12, False, "none", True), (8192, SWEEP_POINTS >> 13, False, "none", True),
+    (1 << 15, SWEEP_POINTS >> 15, False, "none", True), (1 << 18, SWEEP_POINTS >> 18, False, "none", True),
+    (4096, SWEEP_POINTS >> 12, False, "none", False),
 ]
 
 
 def _form_id(shape, form):
-    n, b, d, _ = shape
-    return f"{form}_{n}x{b}{'_d' if d else '_f'}"
+    n, b, d, _, fixed = shape
+    return f"{form}_{n}x{b}{'_d' if d else '_f'}{'' if fixed else '_generic'}"
 
 
 @pytest.mark.parametrize("shape,form", [(s, f) for s in FORM_SHAPES for f in FORMS],
                          ids=[_form_id(s, f) for s in FORM_SHAPES for f in FORMS])
 def test_forced_tuning_form(shape, form, monkeypatch):
-    n, batch, double, norm = shape
+    n, batch, double, norm, fixed = shape
     monkeypatch.setenv("SYCLFFT_HIP_TUNE", form)
+    if not fixed:
+        monkeypatch.setenv("SYCLFFT_HIP_FIXED", "0")
     expect = None
-    if form == "s0w0p1g0" and n == 4096 and batch > 4096:
-        # many more tiles than resident workgroups (8192 runs its length-specialized kernel,
-        # which has no persistent form)
-        expect = "persistent"
+    if form == "s0w0p1g0" and not fixed:
+        expect = "persistent"  # many more tiles than resident workgroups
     check_plan(n, batch, double, -1, norm, "valu", measure=False, roundtrip=(norm == "orthogonal"),
                seed=20260102, expect=expect)
 
 
-def test_forced_forms_reach_the_kernels():
-    """The forced forms are not no-ops: the persistent form and the multi-launch form each
-    change the launch sequence of a shape where they apply."""
+def test_forced_forms_reach_the_kernels(monkeypatch):
+    """The forced forms are not no-ops: the persistent form (generic kernel) and the
+    multi-launch form each change the launch sequence of a shape where they apply."""
     import os
 
+    monkeypatch.setenv("SYCLFFT_HIP_FIXED", "0")
     descs = {}
     for form in ("s0w0p0g0", "s0w0p1g0", "s1w0p0g0"):
         os.environ["SYCLFFT_HIP_TUNE"] = form
