@@ -59,7 +59,8 @@ enum sfh_flags : uint32_t
     SFH_CONJ_IN = 1u << 0,   // conjugate every loaded element (backward transforms)
     SFH_WFAST = 1u << 4,     // lane mapping: consecutive lanes take consecutive columns
     SFH_LOAD_LDS = 1u << 5,  // load through a coalescing LDS loop instead of into pass 0
-    SFH_STORE_LDS = 1u << 6  // store through a coalescing LDS loop instead of from the last pass
+    SFH_STORE_LDS = 1u << 6, // store through a coalescing LDS loop instead of from the last pass
+    SFH_TILE_REVERSE = 1u << 7 // workgroup g runs tile ntiles - 1 - g (last tiles first)
 };
 
 // Dimension of a launch's tile grid that enumerates transforms (for unit tables).

@@ -1118,6 +1118,11 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
             }
             launch l = make_launch(p, SFH_MI_PLAIN, g, BUF_SCRATCH, dst, false, "fourstep-B");
             l.dst_off = off;
+            // Last tiles first: pass A wrote them last, so part of them is still in the
+            // 256 MiB Infinity Cache when pass B starts (measured 3-5% at 2^19-2^20 points,
+            // neutral below).
+            l.args.flags |= SFH_TILE_REVERSE;
+            l.desc += " reverse";
             p.launches.push_back(l);
         }
 }

@@ -1348,6 +1348,9 @@ __device__ __forceinline__ void store_via_lds(const sfh_tile_args &a, const tile
 template <typename T, int MODE>
 __device__ __forceinline__ void set_tile(const sfh_tile_args &a, tile_ctx<T> &c, uint64_t g, uint32_t tid)
 {
+    if (a.flags & SFH_TILE_REVERSE) {
+        g = a.ntiles - 1 - g;
+    }
     const uint64_t wt = g % a.nwt;
     const uint64_t rest = g / a.nwt;
     const uint64_t blo = rest % a.nblo;
