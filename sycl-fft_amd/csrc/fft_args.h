@@ -142,6 +142,11 @@ struct sfh_tile_args
     void *pair_buf;
     uint32_t *pair_count;
     uint32_t pair_stride;
+    // The plan's table block: the length-specialized kernels read their tables (pass
+    // twiddles, chirps, spectra) from global memory, pass after pass, and touch this block
+    // at their start so that it is in L2 by the time later passes need it (null: no touch).
+    const void *warm;
+    uint64_t warm_bytes;
     sfh_pass pass[SFH_MAX_PASSES];
 };
 

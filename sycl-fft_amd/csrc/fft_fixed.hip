@@ -47,6 +47,10 @@ const sfh_fixed_kernel *table(int *count)
         // the same convolution over a pair of workgroups per transform: half-length FFTs of
         // 10080 = 2^5 3^2 5 7 (n <= 10080), for batches too small to fill the device
         entry<float, 1024, 4, BLUE | SFH_M_PAIR, false, false, true, radices<9, 5, 7, 4, 8>, radices<8, 9, 5, 7, 4>>(),
+        // half length 10240 = 2^11 5 (n <= 10240): four passes per FFT instead of five, with
+        // rounds that fill the 1024 work-items (640, 640, 1280 and 2048 butterflies); the
+        // planner takes it over the smallest 7-smooth half length when it is at most 3% longer
+        entry<float, 1024, 4, BLUE | SFH_M_PAIR, true, false, false, radices<5, 16, 16, 8>, radices<8, 16, 16, 5>>(),
         // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 85 VGPRs, four workgroups per CU
         entry<double, 384, 6, 0, false, true, false, radices<9, 5, 7, 8>, none>(),
         // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB

@@ -340,6 +340,12 @@ __device__ __forceinline__ cx<T> coherent_get(const cx<T> *p)
     return v;
 }
 
+// The store paths below issue all of a butterfly group's loads (tables, partner partials)
+// before its first store: CDNA counts vector loads and stores on one in-order counter, so a
+// load placed after a store waits for that store's completion too, and the compiler cannot
+// hoist a table load above a store to a buffer that might alias it.  Interleaved, every
+// point would cost a full memory round trip.
+
 // Two workgroups per transform (SFH_M_PAIR): half h of transform b holds its partial result
 // P_h[k] = v[k] post[2k + h] (k < n) in registers.  Both halves store theirs to slot 2b + h
 // of pair_buf with coherent stores, wait until every store of the workgroup has completed
@@ -360,6 +366,16 @@ __device__ __forceinline__ void store_pair(const sfh_tile_args &a, const tile_ct
     cx<T> *const mine = pair + (2 * static_cast<uint64_t>(b) + h) * a.pair_stride;
     const cx<T> *const other = pair + (2 * static_cast<uint64_t>(b) + (h ^ 1u)) * a.pair_stride;
     const cx<T> *post = static_cast<const cx<T> *>(a.post);
+    cx<T> pv[sh::NB][R];
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const uint32_t pq = (i + static_cast<uint32_t>(t * M)) * a.post_kmul + c.col * a.post_cmul;
+            pv[q][t] = (sh::has(q, i) && pq < a.nvalid_out) ? post[pq] : cx<T>{T(0), T(0)};
+        }
+    }
 #pragma unroll
     for (int q = 0; q < sh::NB; ++q) {
         const uint32_t i = tid + static_cast<uint32_t>(q * NT);
@@ -369,7 +385,7 @@ __device__ __forceinline__ void store_pair(const sfh_tile_args &a, const tile_ct
                 const uint32_t k = i + static_cast<uint32_t>(t * M);
                 const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
                 if (pq < a.nvalid_out) {
-                    r.v[q][t] = cmul(r.v[q][t], post[pq]);
+                    r.v[q][t] = cmul(r.v[q][t], pv[q][t]);
                     coherent_put(mine + k, r.v[q][t]);
                 }
             }
@@ -390,12 +406,19 @@ __device__ __forceinline__ void store_pair(const sfh_tile_args &a, const tile_ct
     for (int q = 0; q < sh::NB; ++q) {
         const uint32_t i = tid + static_cast<uint32_t>(q * NT);
         if (sh::has(q, i)) {
+            cx<T> o[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const uint32_t k = i + static_cast<uint32_t>(t * M);
+                const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
+                o[t] = pq < a.nvalid_out ? coherent_get(other + k) : cx<T>{T(0), T(0)};
+            }
 #pragma unroll
             for (int t = 0; t < R; ++t) {
                 const uint32_t k = i + static_cast<uint32_t>(t * M);
                 const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
                 if (pq < a.nvalid_out) {
-                    out_global(c.dst)[k] = ep(r.v[q][t] + coherent_get(other + k));
+                    out_global(c.dst)[k] = ep(r.v[q][t] + o[t]);
                 }
             }
         }
@@ -410,20 +433,39 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
     constexpr int R = sh::R, M = sh::M;
     const epilogue<T> ep(a);
     const uint32_t tid = threadIdx.x;
+    if constexpr (MODE & SFH_M_POST) {
+        cx<T> pv[sh::NB][R];
 #pragma unroll
-    for (int q = 0; q < sh::NB; ++q) {
-        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
-        if (sh::has(q, i)) {
+        for (int q = 0; q < sh::NB; ++q) {
+            const uint32_t i = tid + static_cast<uint32_t>(q * NT);
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const uint32_t k = i + static_cast<uint32_t>(t * M);
-                if constexpr (MODE & SFH_M_POST) {
-                    const uint32_t pq = k * a.post_kmul + c.col * a.post_cmul;
-                    if (pq < a.nvalid_out) {
-                        out_global(c.dst)[k] = ep(cmul(r.v[q][t], static_cast<const cx<T> *>(a.post)[pq]));
+                const uint32_t pq = (i + static_cast<uint32_t>(t * M)) * a.post_kmul + c.col * a.post_cmul;
+                pv[q][t] = (sh::has(q, i) && pq < a.nvalid_out) ? static_cast<const cx<T> *>(a.post)[pq]
+                                                                : cx<T>{T(0), T(0)};
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < sh::NB; ++q) {
+            const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+            if (sh::has(q, i)) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const uint32_t k = i + static_cast<uint32_t>(t * M);
+                    if (k * a.post_kmul + c.col * a.post_cmul < a.nvalid_out) {
+                        out_global(c.dst)[k] = ep(cmul(r.v[q][t], pv[q][t]));
                     }
-                } else {
-                    out_global(c.dst)[k] = ep(r.v[q][t]);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < sh::NB; ++q) {
+            const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+            if (sh::has(q, i)) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    out_global(c.dst)[i + static_cast<uint32_t>(t * M)] = ep(r.v[q][t]);
                 }
             }
         }
@@ -546,6 +588,28 @@ __device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<
     }
 }
 
+// Touch this workgroup's share of the plan's table block, one 4-byte load per 128-byte
+// line: the lines are split among the workgroups that share an XCD (workgroups are dealt
+// to the 8 XCDs round robin, so those of one XCD are g, g + 8, ...), so each XCD's L2 holds
+// the block after the first tiles' loads.  Returns a value the caller keeps alive to the
+// end of the kernel, so that the loads are not removed.
+__device__ __forceinline__ uint32_t warm_tables(const sfh_tile_args &a)
+{
+    uint32_t acc = 0;
+    if (a.warm != nullptr) {
+        const uint64_t lines = (a.warm_bytes + 127) / 128;
+        const uint64_t groups = gridDim.x / 8 > 0 ? gridDim.x / 8 : 1;
+        const uint64_t per = (lines + groups - 1) / groups;
+        const uint64_t l0 = (blockIdx.x / 8) % groups * per;
+        const uint64_t l1 = l0 + per < lines ? l0 + per : lines;
+        const uint32_t *w = static_cast<const uint32_t *>(a.warm);
+        for (uint64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+            acc += w[l * 32];
+        }
+    }
+    return acc;
+}
+
 } // namespace fx
 
 // Entry: tile g = blockIdx.x is one transform (W = 1), addressed as by the generic kernel.
@@ -569,9 +633,15 @@ void fixed_fft(const sfh_tile_args a_byval)
     SFH_DYNAMIC_LDS(sfh_smem);
     tile_ctx<T> c;
     set_tile<T, MODE>(a, c, blockIdx.x, threadIdx.x);
+    const uint32_t warmed = fx::warm_tables(a);
     fx::regs<T, NT, L, prog, 0> r;
     const fx::twregs<T, NT, L, prog, 0> w0{}; // the first step reads HBM: no pass twiddles
     fx::run_from<SPLIT, PF, T, NT, L, MODE, prog, 0>(a, c, sfh_smem, static_cast<const cx<T> *>(a.twtab), r, w0);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::"v"(warmed)); // the touch loads stay (their values are not needed)
+#else
+    (void)warmed;
+#endif
 }
 
 } // namespace sfh

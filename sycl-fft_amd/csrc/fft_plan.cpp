@@ -1137,14 +1137,35 @@ static void plan_fourstep(plan &p, uint64_t outer, uint64_t n, uint64_t inner, u
 // writes P_h[m] = c[m] E_0[m] and c[m] W_M^-m E_1[m] (m < n) to scratch; launch 2, a radix-2
 // tile of the pair whose difference output is skipped (POST bound 1), writes
 // y[m] = P_0[m] + P_1[m].
-static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id src, buf_id dst)
+// Half length of the Bluestein pair for n: the smallest 7-smooth Mh >= n that fits a tile,
+// or a longer one (at most 3% longer) that has a length-specialized pair kernel with fewer
+// passes; 0 if none fits.
+static uint64_t pair_half_length(const plan &p, uint64_t n, const sfh_fixed_kernel **kernel)
 {
     const uint64_t cap = static_cast<uint64_t>(max_cap(p));
     const uint64_t Mh = smooth_at_least(n, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
+    *kernel = Mh ? find_fixed(p, SFH_FX_BLUE_PAIR, Mh) : nullptr;
+    int nk = 0;
+    const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    for (int i = 0; i < nk && Mh; ++i) {
+        const sfh_fixed_kernel *k = &ks[i];
+        if (k->mode_index == SFH_FX_BLUE_PAIR && k->is_double == (p.dbl ? 1 : 0) && k->L >= n && k->L > Mh
+            && 100 * static_cast<uint64_t>(k->L) <= 103 * Mh && k->lds_bytes <= k_lds_limit
+            && (!*kernel || k->n1 < (*kernel)->n1) && find_fixed(p, SFH_FX_BLUE_PAIR, k->L) == k) {
+            *kernel = k;
+            return k->L;
+        }
+    }
+    return Mh;
+}
+
+static bool plan_bluestein_halves(plan &p, uint64_t outer, uint64_t n, buf_id src, buf_id dst)
+{
+    const sfh_fixed_kernel *pair = nullptr;
+    const uint64_t Mh = pair_half_length(p, n, &pair);
     if (!Mh) {
         return false;
     }
-    const sfh_fixed_kernel *pair = find_fixed(p, SFH_FX_BLUE_PAIR, Mh);
     const uint64_t M = 2 * Mh;
     std::vector<cxl> chirp(n), bvec(M, cxl{0.0L, 0.0L});
     for (uint64_t j = 0; j < n; ++j) {
@@ -1300,9 +1321,9 @@ static void plan_bluestein(plan &p, uint64_t outer, uint64_t n, uint64_t inner, 
         bool take = force == 1;
         if (force < 0) {
             const uint64_t cus = static_cast<uint64_t>(std::max(0, sfh_device_cus()));
-            const uint64_t cap = static_cast<uint64_t>(max_cap(p));
-            const uint64_t Mh = smooth_at_least(n, [&](uint64_t m) { return m <= cap && fits_single(p, m, 1); });
-            take = cus > 0 && 2 * outer <= cus && Mh && find_fixed(p, SFH_FX_BLUE_PAIR, Mh) != nullptr;
+            const sfh_fixed_kernel *pair = nullptr;
+            const uint64_t Mh = pair_half_length(p, n, &pair);
+            take = cus > 0 && 2 * outer <= cus && Mh && pair != nullptr;
         }
         if (take && plan_bluestein_halves(p, outer, n, src, dst)) {
             return;
@@ -2000,6 +2021,15 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
         l.args.mid = ptr(l.mid_off);
         l.args.post = ptr(l.post_off);
         l.fx_twtab = ptr(l.fx_tw_off);
+        if (l.fixed && p->tables) {
+            // the length-specialized kernels read their tables from global memory pass after
+            // pass: they touch the block at their start (SYCLFFT_HIP_WARM=0: they do not)
+            const char *we = std::getenv("SYCLFFT_HIP_WARM");
+            if (!we || std::strcmp(we, "0") != 0) {
+                l.args.warm = p->tables;
+                l.args.warm_bytes = p->tables_bytes;
+            }
+        }
     }
     p->host_tables.clear();
     p->host_tables.shrink_to_fit();

@@ -1081,6 +1081,24 @@ __device__ __forceinline__ void run_pass(
         // multiplied by post[pq]
         const uint32_t qstep = m * a.post_kmul;
         const cx<T> *post = static_cast<const cx<T> *>(a.post);
+        if constexpr (MODE & SFH_M_POST) {
+            // every post-table load before the first store (one in-order counter for vector
+            // loads and stores: a load after a store would also wait for the store)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
+                if (wvalid && i < m) {
+                    const uint32_t qq = i * a.post_kmul + c.col * a.post_cmul;
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const uint32_t pq = qq + static_cast<uint32_t>(t) * qstep;
+                        if (pq < a.nvalid_out) {
+                            v[q][t] = cmul(v[q][t], post[pq]);
+                        }
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const uint32_t i = c.lt + static_cast<uint32_t>(q) * a.tpc;
@@ -1092,7 +1110,7 @@ __device__ __forceinline__ void run_pass(
                     for (int t = 0; t < R; ++t) {
                         const uint32_t pq = qq + static_cast<uint32_t>(t) * qstep;
                         if (pq < a.nvalid_out) {
-                            bq[t * tstep] = ep(cmul(v[q][t], post[pq]));
+                            bq[t * tstep] = ep(v[q][t]);
                         }
                     }
                 } else {
@@ -1324,6 +1342,55 @@ template <typename T, int NT, int E, int MODE>
 __device__ __forceinline__ void store_via_lds(const sfh_tile_args &a, const tile_ctx<T> &c, const cx<T> *lds)
 {
     const bool wcontig = (a.out_w == 1) && a.W > 1;
+    if constexpr (MODE & SFH_M_POST) {
+        // every post-table load before the first store: vector loads and stores share one
+        // in-order completion counter, so a load after a store would wait for the store
+        cx<T> pv[E];
+        const cx<T> *post = static_cast<const cx<T> *>(a.post);
+#pragma unroll
+        for (int n = 0; n < E; ++n) {
+            const uint32_t e = threadIdx.x + static_cast<uint32_t>(n) * NT;
+            uint32_t w, k;
+            if (wcontig) {
+                w = e & (a.W - 1);
+                k = e >> a.logW;
+            } else {
+                w = fdiv(e, a.L_div);
+                k = e - w * a.L;
+            }
+            pv[n] = {T(0), T(0)};
+            if (k < a.L && w < c.wv) {
+                const uint32_t q = k * a.post_kmul + col_of(a, c.w0, c.blo, w) * a.post_cmul;
+                if (q < a.nvalid_out) {
+                    pv[n] = post[q];
+                }
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < E; ++n) {
+            const uint32_t e = threadIdx.x + static_cast<uint32_t>(n) * NT;
+            uint32_t w, k;
+            if (wcontig) {
+                w = e & (a.W - 1);
+                k = e >> a.logW;
+            } else {
+                w = fdiv(e, a.L_div);
+                k = e - w * a.L;
+            }
+            if (k < a.L && w < c.wv) {
+                const uint32_t col = col_of(a, c.w0, c.blo, w);
+                if (k * a.post_kmul + col * a.post_cmul < a.nvalid_out) {
+                    cx<T> v = lds[w * a.LP + lds_slot(k, a.pad_store)];
+                    if constexpr (MODE & SFH_M_T4) {
+                        v = cmul(v, t4_twiddle(a, c.t4, col * k));
+                    }
+                    cx<T> *colp = c.tdst + static_cast<int32_t>(w) * a.out_w;
+                    colp[static_cast<int32_t>(k) * a.out_k] = epilogue<T>(a)(cmul(v, pv[n]));
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < E; ++n) {
         const uint32_t e = threadIdx.x + static_cast<uint32_t>(n) * NT;
