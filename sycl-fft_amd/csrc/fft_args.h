@@ -50,7 +50,10 @@ enum sfh_mode : int
     // length-specialized Bluestein over two workgroups per transform (with PRE | MID | POST):
     // each writes its half's partial result to pair_buf, and the second of the pair to finish
     // adds the two and writes the output
-    SFH_M_PAIR = 64
+    SFH_M_PAIR = 64,
+    // length-specialized kernels: two LDS buffers used in turn by successive exchanges, so an
+    // exchange needs one barrier (between its writes and reads) instead of two
+    SFH_M_DBUF = 128
 };
 
 // Run-time flags.

@@ -21,7 +21,7 @@ sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
     k.nt = NT;
     k.L = static_cast<uint32_t>(prog::L);
     // LDS bytes per tile: one complex value per slot, or one real with the split exchange
-    k.lds_bytes = static_cast<uint32_t>(prog::slots() * (SPLIT ? sizeof(T) : 2 * sizeof(T)));
+    k.lds_bytes = static_cast<uint32_t>(prog::slots() * (SPLIT ? sizeof(T) : 2 * sizeof(T)) * ((MODE & SFH_M_DBUF) ? 2 : 1));
     k.max_tiles_per_cu = max_tiles_per_cu;
     k.n1 = R1::n;
     for (int i = 0; i < R1::n; ++i) {
@@ -44,13 +44,15 @@ const sfh_fixed_kernel *table(int *count)
         // e.g. the prime 10007): odd radices first, so that no power-of-two pass writes at a
         // small ns (bank conflicts); 157.5 KiB of LDS, no room for the pad
         entry<float, 1024, 4, BLUE, false, false, false, radices<9, 5, 7, 8, 8>, radices<8, 9, 5, 7, 8>>(),
-        // the same convolution over a pair of workgroups per transform: half-length FFTs of
-        // 10080 = 2^5 3^2 5 7 (n <= 10080), for batches too small to fill the device
-        entry<float, 1024, 4, BLUE | SFH_M_PAIR, false, false, true, radices<9, 5, 7, 4, 8>, radices<8, 9, 5, 7, 4>>(),
-        // half length 10240 = 2^11 5 (n <= 10240): four passes per FFT instead of five, with
-        // rounds that fill the 1024 work-items (640, 640, 1280 and 2048 butterflies); the
-        // planner takes it over the smallest 7-smooth half length when it is at most 3% longer
-        entry<float, 1024, 4, BLUE | SFH_M_PAIR, true, false, false, radices<5, 16, 16, 8>, radices<8, 16, 16, 5>>(),
+        // the convolution over a pair of workgroups per transform, for batches too small to fill
+        // the device: half-length FFTs of 10240 = 2^11 5 (n <= 10240).  The planner takes it
+        // over the smallest 7-smooth half length when it is at most 3% longer (n = 10007: 10080,
+        // 9 5 7 4 8, five passes with ragged rounds): four passes whose butterfly rounds fill
+        // the 1024 work-items (2048, 640, 640 and 1280 butterflies), measured 35.0 -> 31.7 us at
+        // batch 128; the two LDS buffers (160 KiB, one workgroup per CU either way) save one
+        // barrier per exchange, 31.7 -> 29.7 us
+        entry<float, 1024, 4, BLUE | SFH_M_PAIR | SFH_M_DBUF, false, false, false, radices<5, 16, 16, 8>,
+              radices<8, 16, 16, 5>>(),
         // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 85 VGPRs, four workgroups per CU
         entry<double, 384, 6, 0, false, true, false, radices<9, 5, 7, 8>, none>(),
         // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB
@@ -61,8 +63,9 @@ const sfh_fixed_kernel *table(int *count)
         // (more independent butterflies per wave between barriers) ran 2-5% faster at every
         // length; the register file then bounds the tiles per CU (about 256 KiB of points).
         // 4096 for batches of at most one transform per CU (latency-bound single round): 512
-        // work-items per transform, every pass boundary pad-linear
-        entry<float, 512, 2, 0, true, false, false, radices<4, 8, 16, 8>, none>(1),
+        // work-items per transform, every pass boundary pad-linear, two LDS buffers (one
+        // barrier per exchange)
+        entry<float, 512, 2, SFH_M_DBUF, true, false, false, radices<4, 8, 16, 8>, none>(1),
         // 4096, larger batches: 128 work-items x 32 points, split tiles of 16.5 KiB
         entry<float, 128, 4, 0, true, true, false, radices<4, 8, 16, 8>, none>(),
         // 8192: the radix-2 pass between two radix-16 passes keeps every pass boundary

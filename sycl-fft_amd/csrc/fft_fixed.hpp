@@ -542,10 +542,21 @@ __device__ __forceinline__ void read_in(const unsigned char *lds, regs<T, NT, L,
 // in registers on both sides, so with SPLIT the real parts move first and the imaginary
 // parts second through an LDS buffer of one real per point (half the footprint, twice the
 // barriers).  The leading barrier ends the previous exchange's reads (none before the first).
-template <bool SPLIT, typename T, int NT, int L, class PROG, int S>
+// With DBUF, exchange S uses buffer S mod 2: the previous exchange read the other buffer, and
+// the exchange before it (same buffer) ended its reads before the previous exchange's
+// barrier, so no leading barrier is needed.
+template <bool SPLIT, bool DBUF, typename T, int NT, int L, class PROG, int S>
 __device__ __forceinline__ void exchange(unsigned char *lds, const regs<T, NT, L, PROG, S> &out,
                                          regs<T, NT, L, PROG, S + 1> &in)
 {
+    if constexpr (DBUF) {
+        static_assert(!SPLIT, "the double-buffered exchange moves whole complex values");
+        unsigned char *buf = lds + (S % 2) * PROG::slots() * sizeof(cx<T>);
+        write_out<2>(buf, out);
+        __syncthreads();
+        read_in<2>(buf, in);
+        return;
+    }
     if constexpr (S > 0) {
         __syncthreads();
     }
@@ -583,7 +594,7 @@ __device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<
         }
         compute<PF, T, NT, L, MODE, PROG, S>(a, c, tw, w, r);
         regs<T, NT, L, PROG, S + 1> next;
-        exchange<SPLIT>(lds, r, next);
+        exchange<SPLIT, (MODE & SFH_M_DBUF) != 0>(lds, r, next);
         run_from<SPLIT, PF, T, NT, L, MODE, PROG, S + 1>(a, c, lds, tw, next, wn);
     }
 }
