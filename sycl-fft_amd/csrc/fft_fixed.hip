@@ -71,7 +71,10 @@ const sfh_fixed_kernel *table(int *count)
         // 8192: the radix-2 pass between two radix-16 passes keeps every pass boundary
         // pad-linear (no bank conflicts)
         entry<float, 256, 2, 0, true, true, false, radices<16, 2, 16, 16>, none>(),
-        entry<float, 256, 2, 0, true, true, false, radices<16, 16, 16, 4>, none>(),
+        // 16384: 512 work-items x 32 points (two tiles per CU, 106 VGPRs).  Measured at 512 MiB
+        // against 256 x 64 (16 16 16 4, 223 VGPRs): 0.54 against 0.50-0.52 of the HBM peak; the
+        // radix orders 8 16 16 8 and 16 8 16 8 ran within 1%, twiddle prefetch spilled (0.44)
+        entry<float, 512, 4, 0, true, true, false, radices<16, 8, 8, 16>, none>(),
         // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
         // the 64 points of each work-item stay in registers across every exchange
         entry<float, 512, 2, 0, true, true, false, radices<16, 16, 16, 8>, none>(),

@@ -1768,7 +1768,14 @@ static void choose_fixed(plan &p)
     int nk = 0;
     const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
     const uint64_t cus = static_cast<uint64_t>(std::max(0, sfh_device_cus()));
+    // SYCLFFT_HIP_FIXED_SKIP=<k>: pass over the first k qualifying entries of each launch
+    // (a test and measurement knob: it reaches the entries listed after the preferred one)
+    int skip0 = 0;
+    if (const char *e = std::getenv("SYCLFFT_HIP_FIXED_SKIP")) {
+        skip0 = std::atoi(e);
+    }
     for (auto &l : p.launches) {
+        int skip = skip0;
         const sfh_tile_args &a = l.args;
         if (l.fixed || a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0
             || l.grid < a.ntiles || (l.mode != SFH_MI_PLAIN && l.mode != SFH_MI_BLUE_FUSED)) {
@@ -1779,6 +1786,9 @@ static void choose_fixed(plan &p)
             if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L && k.lds_bytes <= k_lds_limit
                 && (k.max_tiles_per_cu == 0
                     || (cus > 0 && a.ntiles <= cus * static_cast<uint64_t>(k.max_tiles_per_cu)))) {
+                if (skip-- > 0) {
+                    continue;
+                }
                 l.fixed = &k;
                 l.fx_tw_off = fixed_tables(p, k);
                 l.desc += " fixed=" + std::to_string(k.nt) + "x" + std::to_string(k.L);

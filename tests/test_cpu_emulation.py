@@ -156,7 +156,7 @@ def test_emulated_work_queue_execution():
 def test_emulated_bluestein_pair():
     """The length-specialized Bluestein kernel over two workgroups per transform (forced with
     SYCLFFT_HIP_BLUE_HALVES=1 where the emulation reports no CU count): every direction and
-    normalization, in place, primes whose half length is 10080, contiguous and over pointer
+    normalization, in place, lengths whose half length is 10240, contiguous and over pointer
     arrays (the second half of each pair to finish adds the two partial results)."""
     env = {"SYCLFFT_HIP_BLUE_HALVES": "1"}
     cases = ["f:10007:3:-1:3:0", "f:10007:2:1:0:1", "f:9973:2:-1:1:0", "f:10079:1:1:2:0"]

@@ -132,7 +132,7 @@ def test_repeated_execution_serializes():
 
 @pytest.mark.parametrize("in_place", [False, True])
 def test_bluestein_pair_repeats(in_place, monkeypatch):
-    """The Bluestein form over two workgroups per transform (n = 10007, half length 10080):
+    """The Bluestein form over two workgroups per transform (n = 10007, half length 10240):
     the second workgroup of each pair to finish adds the halves, counting arrivals on a
     per-transform counter that every execution advances by two.  Repeated executions must
     give bit-identical results, within the oracle bound."""
