@@ -217,6 +217,8 @@ struct launch
     // length-specialized kernel replacing the generic entry (null: generic), with its
     // per-pass twiddle tables (offset into the plan's table block, then device address)
     const sfh_fixed_kernel *fixed{nullptr};
+    // no generic variant holds this launch's tile: only `fixed` can run it
+    bool fixed_only{false};
     int64_t fx_tw_off{-1};
     const void *fx_twtab{nullptr};
     std::string desc;
@@ -875,6 +877,7 @@ static launch make_fixed_launch(plan &p, const sfh_fixed_kernel &k, const geomet
     l.src = src;
     l.dst = dst;
     l.variant = 0; // unused: the fixed kernel runs every execution of this launch
+    l.fixed_only = true;
     sfh_tile_args &a = l.args;
     a.L = static_cast<uint32_t>(g.L);
     a.LP = a.L;
@@ -1801,6 +1804,9 @@ static void choose_fixed(plan &p)
 // Back to the generic kernel for a launch whose arguments leave the specialized contract.
 static void unfix(launch &l)
 {
+    if (l.fixed_only) {
+        throw sfh_error(SYCLFFT_HIP_STATUS_PLANNING_FAILED, "launch has no generic kernel form");
+    }
     if (l.fixed) {
         l.fixed = nullptr;
         const size_t at = l.desc.find(" fixed=");
@@ -2031,9 +2037,12 @@ static plan *build_plan(const syclfft_hip_plan_config_v1 *cfg, int dev, const tu
         l.args.mid = ptr(l.mid_off);
         l.args.post = ptr(l.post_off);
         l.fx_twtab = ptr(l.fx_tw_off);
-        if (l.fixed && p->tables) {
-            // the length-specialized kernels read their tables from global memory pass after
-            // pass: they touch the block at their start (SYCLFFT_HIP_WARM=0: they do not)
+        if (l.fixed && l.fixed->mode_index != SFH_MI_PLAIN && p->tables) {
+            // the length-specialized Bluestein kernels read their tables (pass twiddles of two
+            // FFTs, chirps, spectrum) from global memory over eight to ten passes: they touch
+            // the block at their start (SYCLFFT_HIP_WARM=0: they do not).  Measured on
+            // 10007 x 128 in the round-trip step: 48.8 -> 47.5 us per launch; the plain
+            // kernels, whose few table lines the first tiles read anyway, lost 0.4-0.8 us to it
             const char *we = std::getenv("SYCLFFT_HIP_WARM");
             if (!we || std::strcmp(we, "0") != 0) {
                 l.args.warm = p->tables;
@@ -3133,8 +3142,11 @@ static plan *create_layout_plan(const syclfft_hip_plan_config_v1 *cfg, const syc
         if (!c.copy_only && c.launches.size() == 1) {
             launch &l = c.launches[0];
             sfh_tile_args &a = l.args;
+            // a launch only a length-specialized kernel can run keeps contiguous points: strided
+            // layouts go through the gather / scatter path
             const bool contiguous = l.src == BUF_IN && l.dst == BUF_OUT && l.src_off == 0 && l.dst_off == 0
                                     && a.nblo == 1 && a.in_j == 1 && a.out_k == 1
+                                    && (!l.fixed_only || (lay->in_stride == 1 && lay->out_stride == 1))
                                     && static_cast<uint64_t>(a.in_w) == n && static_cast<uint64_t>(a.out_w) == n
                                     && a.in_units == nullptr && a.out_units == nullptr;
             const auto span = [&](uint64_t sw, uint64_t sj) { return sw * (a.W - 1) + sj * (a.L - 1); };

@@ -157,6 +157,26 @@ def test_bluestein_pair_repeats(in_place, monkeypatch):
     assert rel_l2(outs[0], oracle_fftn(x, [n], b, 1, "orthogonal")) < tol_l2(n, False, 4.0)
 
 
+@pytest.mark.parametrize("n,b,direction,norm", [
+    (10007, 300, -1, "orthogonal"), (10007, 257, 1, "none"), (9973, 200, -1, "forward"), (7919, 160, 1, "backward"),
+])
+def test_bluestein_one_workgroup_large_batch(n, b, direction, norm):
+    """Bluestein with one workgroup per transform (batches above half the CU count): n = 10007
+    runs the length-specialized 20160-point convolution, 9973 and 7919 the generic kernel.
+    Against the oracle."""
+    x = rnd(n * b, 47)
+    say(f"case bluestein one workgroup n={n} b={b}")
+    p = sh.Plan([n], b, direction, normalization=norm)
+    desc = p.describe()
+    assert desc.startswith("bluestein-fused"), desc
+    xd = dev_tensor(x, False)
+    yd = torch.empty_like(xd)
+    p.execute(xd, yd)
+    torch.cuda.synchronize()
+    p.close()
+    assert rel_l2(yd.cpu().numpy(), oracle_fftn(x, [n], b, direction, norm)) < tol_l2(n, False, 4.0)
+
+
 def test_concurrent_plans_and_impulse():
     """Two plans on one stream, impulse inputs (tests/sycl_tests.cpp:410-435)."""
     say("case concurrent plans")
