@@ -66,18 +66,23 @@ const sfh_fixed_kernel *table(int *count)
         // work-items per transform, every pass boundary pad-linear, two LDS buffers (one
         // barrier per exchange)
         entry<float, 512, 2, SFH_M_DBUF, true, false, false, radices<4, 8, 16, 8>, none>(1),
-        // 4096, larger batches: 128 work-items x 32 points, split tiles of 16.5 KiB
-        entry<float, 128, 4, 0, true, true, false, radices<4, 8, 16, 8>, none>(),
-        // 8192: the radix-2 pass between two radix-16 passes keeps every pass boundary
-        // pad-linear (no bank conflicts)
-        entry<float, 256, 2, 0, true, true, false, radices<16, 2, 16, 16>, none>(),
+        // 4096, larger batches: 128 work-items x 32 points, split tiles of 16.5 KiB.  Radix 32
+        // first (32 16 8: two LDS exchanges, every boundary pad-linear) measured 203-214 us per
+        // 512 MiB against 215-218 us for 4 8 16 8 (three exchanges)
+        entry<float, 128, 4, 0, true, true, false, radices<32, 16, 8>, none>(),
+        // 8192: 256 work-items x 32 points, 32 16 16 (two exchanges, pad-linear): 209-210 us
+        // against 214 us for 16 2 16 16
+        entry<float, 256, 2, 0, true, true, false, radices<32, 16, 16>, none>(),
         // 16384: 512 work-items x 32 points (two tiles per CU, 106 VGPRs).  Measured at 512 MiB
         // against 256 x 64 (16 16 16 4, 223 VGPRs): 0.54 against 0.50-0.52 of the HBM peak; the
-        // radix orders 8 16 16 8 and 16 8 16 8 ran within 1%, twiddle prefetch spilled (0.44)
+        // radix orders 8 16 16 8 and 16 8 16 8 ran within 1%, twiddle prefetch spilled (0.44);
+        // with radix 32, 32 16 32 ran within 1% and 32 32 16 12% slower
         entry<float, 512, 4, 0, true, true, false, radices<16, 8, 8, 16>, none>(),
         // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
-        // the 64 points of each work-item stay in registers across every exchange
-        entry<float, 512, 2, 0, true, true, false, radices<16, 16, 16, 8>, none>(),
+        // the 64 points of each work-item stay in registers across every exchange; 32 32 32 (two
+        // exchanges) measured 302 us against 306-307 us for 16 16 16 8, and 1024 x 32 points
+        // 303-307 us
+        entry<float, 512, 2, 0, true, true, false, radices<32, 32, 32>, none>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

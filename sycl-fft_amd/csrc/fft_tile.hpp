@@ -202,6 +202,52 @@ struct dft<T, 16>
     }
 };
 
+// radix 32 as 2 x 16: the even and odd points (n = 2m + r) each take a 16-point DFT, the
+// odd ones are twiddled by w32^k, and radix-2 butterflies give k and k + 16
+template <typename T>
+struct dft<T, 32>
+{
+    static __device__ __forceinline__ void run(cx<T> *v)
+    {
+        const T c1 = T(0.98078528040323044913); // cos(pi/16)
+        const T s1 = T(0.19509032201612826785);
+        const T c2 = T(0.92387953251128675613); // cos(pi/8)
+        const T s2 = T(0.38268343236508977173);
+        const T c3 = T(0.83146961230254523708); // cos(3 pi/16)
+        const T s3 = T(0.55557023301960222474);
+        const T h = T(0.70710678118654752440);
+        cx<T> e[16], o[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            e[m] = v[2 * m];
+            o[m] = v[2 * m + 1];
+        }
+        dft<T, 16>::run(e);
+        dft<T, 16>::run(o);
+        // w32^k = (cos(pi k / 16), -sin(pi k / 16))
+        o[1] = cmul(o[1], cx<T>{c1, -s1});
+        o[2] = cmul(o[2], cx<T>{c2, -s2});
+        o[3] = cmul(o[3], cx<T>{c3, -s3});
+        o[4] = cmul(o[4], cx<T>{h, -h});
+        o[5] = cmul(o[5], cx<T>{s3, -c3});
+        o[6] = cmul(o[6], cx<T>{s2, -c2});
+        o[7] = cmul(o[7], cx<T>{s1, -c1});
+        o[8] = mul_mi(o[8]);
+        o[9] = cmul(o[9], cx<T>{-s1, -c1});
+        o[10] = cmul(o[10], cx<T>{-s2, -c2});
+        o[11] = cmul(o[11], cx<T>{-s3, -c3});
+        o[12] = cmul(o[12], cx<T>{-h, -h});
+        o[13] = cmul(o[13], cx<T>{-c3, -s3});
+        o[14] = cmul(o[14], cx<T>{-c2, -s2});
+        o[15] = cmul(o[15], cx<T>{-c1, -s1});
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            v[k] = e[k] + o[k];
+            v[k + 16] = e[k] - o[k];
+        }
+    }
+};
+
 template <typename T>
 struct dft<T, 3>;
 
