@@ -62,10 +62,12 @@ const sfh_fixed_kernel *table(int *count)
         // against 2-4x the work-items per transform: fewer work-items with more points each
         // (more independent butterflies per wave between barriers) ran 2-5% faster at every
         // length; the register file then bounds the tiles per CU (about 256 KiB of points).
-        // 4096 for batches of at most one transform per CU (latency-bound single round): 512
-        // work-items per transform, every pass boundary pad-linear, two LDS buffers (one
-        // barrier per exchange)
-        entry<float, 512, 2, SFH_M_DBUF, true, false, false, radices<4, 8, 16, 8>, none>(1),
+        // 4096 for batches of at most one transform per CU (latency-bound single round): 256
+        // work-items x 16 points, radix 16 16 16 (two exchanges), two LDS buffers (one barrier
+        // per exchange).  At 4096 x 256, interleaved over two rounds: 10.8-11.0 us back to back
+        // and 14.9 us after a cache scrub, against 11.4-11.7 / 15.4-15.6 us for 512 x 8 points
+        // (4 8 16 8) and 11.0-11.2 / 15.4 us for 256 x 16 (32 16 8)
+        entry<float, 256, 2, SFH_M_DBUF, true, false, false, radices<16, 16, 16>, none>(1),
         // 4096, larger batches: 128 work-items x 32 points, split tiles of 16.5 KiB.  Radix 32
         // first (32 16 8: two LDS exchanges, every boundary pad-linear) measured 203-214 us per
         // 512 MiB against 215-218 us for 4 8 16 8 (three exchanges)
