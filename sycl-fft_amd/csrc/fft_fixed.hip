@@ -53,10 +53,15 @@ const sfh_fixed_kernel *table(int *count)
         // barrier per exchange, 31.7 -> 29.7 us
         entry<float, 1024, 4, BLUE | SFH_M_PAIR | SFH_M_DBUF, false, false, false, radices<5, 16, 16, 8>,
               radices<8, 16, 16, 5>>(),
-        // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 85 VGPRs, four workgroups per CU
-        entry<double, 384, 6, 0, false, true, false, radices<9, 5, 7, 8>, none>(),
+        // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 512 work-items (60 VGPRs), four
+        // workgroups per CU: 32 waves per CU, the occupancy limit.  At 2520 x 1024 (one round of
+        // tiles), interleaved over two rounds: 20.4-20.5 us back to back and 32.4-32.6 us after a
+        // cache scrub, against 22.8-23.1 / 36.6-36.9 us for 384 work-items (78 VGPRs), 21.7-22.0 /
+        // 32.8-33.1 us for 256; the pad layout and the radix order 5 7 9 8 made no difference
+        entry<double, 512, 8, 0, false, true, false, radices<9, 5, 7, 8>, none>(),
         // complex<double> 6561 = 3^8: one radix-9 butterfly per work-item and pass; 52.5 KiB
-        // split tiles, two workgroups per CU
+        // split tiles, two workgroups per CU.  At 6561 x 512: 26.8-27.1 us back to back against
+        // 27.3-27.5 us for 1024 work-items and 33.9 us for 832
         entry<double, 768, 6, 0, false, true, false, radices<9, 9, 9, 9>, none>(),
         // complex<float> powers of two, pad layout where linear.  Measured at 512 MiB batches
         // against 2-4x the work-items per transform: fewer work-items with more points each
