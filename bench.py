@@ -771,7 +771,18 @@ def main():
     # as stale (null) instead.
     traffic, traffic_note = None, "no profile row"
     dom_plan = dom["fwd"].describe().strip().replace("\n", " | ")
-    launches_of = lambda d: d.split(" | tuning ")[0].strip()  # noqa: E731
+    def launches_of(d):
+        """The kernels a plan's launches run.  A length-specialized launch ("fixed=") keeps
+        its label, length, tile count and kernel; the generic variant fields it replaced
+        (threads, elements, LDS) do not run, and measure-mode planning may pick another."""
+        out = []
+        for ln in d.split(" | tuning ")[0].strip().split(" | "):
+            f = ln.split()
+            if any(x.startswith("fixed=") for x in f):
+                f = [x for x in f if not x.startswith(("nt=", "e=", "lds=", "lp="))]
+            out.append(" ".join(f))
+        return out
+
     tfile = os.path.join(ROOT, "profiles", "traffic_c2_roundtrip.txt")
     if os.path.exists(tfile):
         with open(tfile) as f:
