@@ -50,7 +50,9 @@ const sfh_fixed_kernel *table(int *count)
         // 9 5 7 4 8, five passes with ragged rounds): four passes whose butterfly rounds fill
         // the 1024 work-items (2048, 640, 640 and 1280 butterflies), measured 35.0 -> 31.7 us at
         // batch 128; the two LDS buffers (160 KiB, one workgroup per CU either way) save one
-        // barrier per exchange, 31.7 -> 29.7 us
+        // barrier per exchange, 31.7 -> 29.7 us.  Measured against it at batch 128 (event-timed back
+        // to back): twiddle prefetch one pass ahead 35.5-36.3 us (spills), the radix order
+        // 5 8 16 16 / 16 16 8 5 30.5-31.1 us; the pad layout needs more than 160 KiB here
         entry<float, 1024, 4, BLUE | SFH_M_PAIR | SFH_M_DBUF, false, false, false, radices<5, 16, 16, 8>,
               radices<8, 16, 16, 5>>(),
         // complex<double> 2520 = 2^3 3^2 5 7: 20 KiB split tiles, 512 work-items (60 VGPRs), four

@@ -1742,9 +1742,17 @@ static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_
     }
     int nk = 0;
     const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    // SYCLFFT_HIP_FIXED_SKIP=<k> passes over the first k qualifying entries, as in choose_fixed
+    int skip = 0;
+    if (const char *e = std::getenv("SYCLFFT_HIP_FIXED_SKIP")) {
+        skip = std::atoi(e);
+    }
     for (int i = 0; i < nk; ++i) {
         if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L
             && ks[i].lds_bytes <= k_lds_limit && ks[i].max_tiles_per_cu == 0) {
+            if (skip-- > 0) {
+                continue;
+            }
             return &ks[i];
         }
     }
