@@ -27,6 +27,7 @@
 #include <mutex>
 #include <new>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -2206,6 +2207,31 @@ static double time_plan(plan &p)
 
 static std::string describe_plan(const plan &p);
 
+// The launch sequence of a plan description, for the tuner's repeat test.  A
+// length-specialized launch (" fixed=") runs its own kernel whatever generic variant it
+// replaced, so that variant's fields (threads, elements, LDS) are left out: candidates that
+// differ only there run the same kernels and are not timed again.
+static std::string launch_key(const std::string &desc)
+{
+    std::string out, line;
+    std::istringstream in(desc);
+    while (std::getline(in, line)) {
+        if (line.find(" fixed=") != std::string::npos) {
+            std::istringstream words(line);
+            std::string w, kept;
+            while (words >> w) {
+                if (w.rfind("nt=", 0) == 0 || w.rfind("e=", 0) == 0 || w.rfind("lds=", 0) == 0 || w.rfind("lp=", 0) == 0) {
+                    continue;
+                }
+                kept += w + ' ';
+            }
+            line = kept;
+        }
+        out += line + '\n';
+    }
+    return out;
+}
+
 // Greedy search: the default, each single change, then the pairs of improving changes.
 static tuning autotune(const syclfft_hip_plan_config_v1 *cfg, int dev)
 {
@@ -2223,7 +2249,7 @@ static tuning autotune(const syclfft_hip_plan_config_v1 *cfg, int dev)
         // keyed by the launches alone (not the tuning line), so a tuning that yields the same
         // launches as one already timed is not timed again and cannot win on noise
         std::string d = describe_plan(*q);
-        d = d.substr(0, d.rfind("tuning "));
+        d = launch_key(d.substr(0, d.rfind("tuning ")));
         const auto it = seen.find(d);
         if (it != seen.end()) {
             return it->second;
