@@ -92,6 +92,12 @@ const sfh_fixed_kernel *table(int *count)
         // exchanges) measured 302 us against 306-307 us for 16 16 16 8, and 1024 x 32 points
         // 303-307 us
         entry<float, 512, 2, 0, true, true, false, radices<32, 32, 32>, none>(),
+        // complex<double> 4096 and 2048: 16 points per work-item, split tiles (33 and 16.5 KiB).
+        // At 512 MiB batches: 4096 196.5-196.9 us (0.68 of the HBM peak) against 218-219 us on
+        // the generic kernel; 2048 195.6 us against 209.3-209.6 us.  (fp32 2048 on 128 x 16
+        // points or 64 x 32 points tied the generic kernel, 199-222 against 201-217 us.)
+        entry<double, 256, 4, 0, true, true, false, radices<16, 16, 16>, none>(),
+        entry<double, 128, 4, 0, true, true, false, radices<16, 16, 8>, none>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;
