@@ -63,7 +63,10 @@ enum sfh_flags : uint32_t
     SFH_WFAST = 1u << 4,     // lane mapping: consecutive lanes take consecutive columns
     SFH_LOAD_LDS = 1u << 5,  // load through a coalescing LDS loop instead of into pass 0
     SFH_STORE_LDS = 1u << 6, // store through a coalescing LDS loop instead of from the last pass
-    SFH_TILE_REVERSE = 1u << 7 // workgroup g runs tile ntiles - 1 - g (last tiles first)
+    SFH_TILE_REVERSE = 1u << 7, // workgroup g runs tile ntiles - 1 - g (last tiles first)
+    // column-tile length-specialized launches: workgroups g and g + 8 (one XCD, as workgroups
+    // are dealt to the 8 XCDs in turn) run adjacent tiles, whose rows share cache lines
+    SFH_TILE_XCD_PAIR = 1u << 8
 };
 
 // Dimension of a launch's tile grid that enumerates transforms (for unit tables).

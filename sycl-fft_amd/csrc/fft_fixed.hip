@@ -10,18 +10,22 @@ namespace
 {
 using sfh::fx::radices;
 
-template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2>
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2, int W = 1>
 sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
 {
-    using prog = sfh::fx::program<R1, R2, MODE, PAD>;
+    using prog = sfh::fx::program<R1, R2, MODE, PAD, W>;
     sfh_fixed_kernel k{};
-    k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, PF, R1, R2>);
+    k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, PF, R1, R2, W>);
+    k.W = W;
     k.is_double = sizeof(T) == 8;
     k.mode_index = (MODE & SFH_M_PAIR) ? SFH_FX_BLUE_PAIR : ((MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED : SFH_MI_PLAIN);
     k.nt = NT;
     k.L = static_cast<uint32_t>(prog::L);
     // LDS bytes per tile: one complex value per slot, or one real with the split exchange
-    k.lds_bytes = static_cast<uint32_t>(prog::slots() * (SPLIT ? sizeof(T) : 2 * sizeof(T)) * ((MODE & SFH_M_DBUF) ? 2 : 1));
+    // (W columns: W regions of the column pitch)
+    constexpr int elem = static_cast<int>(SPLIT ? sizeof(T) : 2 * sizeof(T));
+    k.lds_bytes = W > 1 ? static_cast<uint32_t>(W * prog::column_pitch(elem) * elem)
+                        : static_cast<uint32_t>(prog::slots() * elem * ((MODE & SFH_M_DBUF) ? 2 : 1));
     k.max_tiles_per_cu = max_tiles_per_cu;
     k.n1 = R1::n;
     for (int i = 0; i < R1::n; ++i) {
@@ -98,6 +102,9 @@ const sfh_fixed_kernel *table(int *count)
         // points or 64 x 32 points tied the generic kernel, 199-222 against 201-217 us.)
         entry<double, 256, 4, 0, true, true, false, radices<16, 16, 16>, none>(),
         entry<double, 128, 4, 0, true, true, false, radices<16, 16, 8>, none>(),
+        // column tiles (strided axes): complex<double> 4096 x 4 columns, 1024 work-items x 16
+        // points (256 per column), split regions of 33 KiB per column
+        entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

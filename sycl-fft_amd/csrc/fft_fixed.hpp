@@ -90,9 +90,15 @@ struct step_info
 // both sides stay linear in the butterfly's point index t: the writer's points
 // base + t ns lie in one 32-point block (ns r divides 32) or ns is a multiple of 32, and
 // the reader's points i + t L / r have L / r a multiple of 32 or a divisor of 32.
-template <class R1, class R2, int MODE, bool PAD>
+template <class R1, class R2, int MODE, bool PAD, int W = 1>
 struct program
 {
+    // columns per tile: W = 1 holds one contiguous transform; W > 1 holds W columns of a
+    // strided axis, lanes column-fastest (lane l takes column l mod W), each column's points
+    // at the launch's point stride, each column in its own LDS region
+    static constexpr int cols = W;
+    static constexpr int log_cols = W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : -1;
+    static_assert(log_cols >= 0, "columns per tile: 1, 2, 4, 8 or 16");
     static constexpr bool two = (MODE & SFH_M_MID) != 0;
     static constexpr int L = length_of<R1>();
     static constexpr int count = two ? R1::n + R2::n - 1 : R1::n;
@@ -147,7 +153,28 @@ struct program
         }
         return any ? L + (L - 1) / 32 + 1 : L;
     }
+    // LDS slots between the regions of two columns (elements of `bytes` each): at least
+    // slots(), and W regions start 64 / W banks (4-byte words, 64 banks) apart, so that the
+    // lanes of one row (consecutive columns) meet different banks
+    static constexpr int column_pitch(int bytes)
+    {
+        if (W == 1) {
+            return slots();
+        }
+        int cp = slots();
+        while ((cp * (bytes / 4)) % 64 != 64 / W) {
+            ++cp;
+        }
+        return cp;
+    }
 };
+
+// The work-item's lane inside its column (the whole workgroup for W = 1).
+template <class PROG>
+__device__ __forceinline__ uint32_t lane()
+{
+    return static_cast<uint32_t>(threadIdx.x) >> PROG::log_cols;
+}
 
 // slot offset of point base + off inside a pad-linear run: off + off / 32
 template <bool PAD>
@@ -186,21 +213,22 @@ __device__ __forceinline__ SFH_GLOBAL_AS cx<T> *out_global(cx<T> *p)
     return (SFH_GLOBAL_AS cx<T> *)p;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int COLS = 1>
 __device__ __forceinline__ cx<T> load_point(const sfh_tile_args &a, const tile_ctx<T> &c, uint32_t j)
 {
+    const uint32_t at = COLS > 1 ? j * static_cast<uint32_t>(a.in_j) : j;
     if constexpr (MODE & SFH_M_PRE) {
         const uint32_t p = j * a.pre_jmul + c.col * a.pre_cmul;
         if (p >= a.nvalid_in) {
             return {T(0), T(0)};
         }
-        cx<T> v = in_global(c.src)[j];
+        cx<T> v = in_global(c.src)[at];
         if (a.flags & SFH_CONJ_IN) {
             v.y = -v.y;
         }
         return cmul(v, static_cast<const cx<T> *>(a.pre)[p]);
     } else {
-        cx<T> v = in_global(c.src)[j];
+        cx<T> v = in_global(c.src)[at];
         if (a.flags & SFH_CONJ_IN) {
             v.y = -v.y;
         }
@@ -248,7 +276,7 @@ __device__ __forceinline__ void load_twiddles(const cx<T> *tw0, twregs<T, NT, L,
     if constexpr (twregs<T, NT, L, PROG, S>::used) {
         constexpr int R = sh::R, NS = sh::st.ns;
         const cx<T> *tw = tw0 + sh::st.twoff;
-        const uint32_t tid = threadIdx.x;
+        const uint32_t tid = lane<PROG>();
 #pragma unroll
         for (int q = 0; q < sh::NB; ++q) {
             const uint32_t i = tid + static_cast<uint32_t>(q * NT);
@@ -274,7 +302,7 @@ __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T
     using sh = shape<NT, L, PROG, S>;
     constexpr step_info st = sh::st;
     constexpr int R = sh::R, M = sh::M, NS = st.ns;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = lane<PROG>();
 #pragma unroll
     for (int q = 0; q < sh::NB; ++q) {
         const uint32_t i = tid + static_cast<uint32_t>(q * NT);
@@ -283,7 +311,7 @@ __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T
             if constexpr (st.in == IN_HBM) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    v[t] = load_point<T, MODE>(a, c, i + static_cast<uint32_t>(t * M));
+                    v[t] = load_point<T, MODE, PROG::cols>(a, c, i + static_cast<uint32_t>(t * M));
                 }
             } else if constexpr (NS > 1) {
                 // prefetched (PF) or loaded here, per butterfly (fewer live registers)
@@ -360,7 +388,7 @@ __device__ __forceinline__ void store_pair(const sfh_tile_args &a, const tile_ct
 {
     using sh = shape<NT, L, PROG, S>;
     constexpr int R = sh::R, M = sh::M;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = lane<PROG>();
     const uint32_t b = c.w0, h = c.blo;
     cx<T> *const pair = static_cast<cx<T> *>(a.pair_buf);
     cx<T> *const mine = pair + (2 * static_cast<uint64_t>(b) + h) * a.pair_stride;
@@ -432,7 +460,7 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
     using sh = shape<NT, L, PROG, S>;
     constexpr int R = sh::R, M = sh::M;
     const epilogue<T> ep(a);
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = lane<PROG>();
     if constexpr (MODE & SFH_M_POST) {
         cx<T> pv[sh::NB][R];
 #pragma unroll
@@ -453,7 +481,8 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
                 for (int t = 0; t < R; ++t) {
                     const uint32_t k = i + static_cast<uint32_t>(t * M);
                     if (k * a.post_kmul + c.col * a.post_cmul < a.nvalid_out) {
-                        out_global(c.dst)[k] = ep(cmul(r.v[q][t], pv[q][t]));
+                        out_global(c.dst)[PROG::cols > 1 ? k * static_cast<uint32_t>(a.out_k) : k] =
+                            ep(cmul(r.v[q][t], pv[q][t]));
                     }
                 }
             }
@@ -465,7 +494,8 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
             if (sh::has(q, i)) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    out_global(c.dst)[i + static_cast<uint32_t>(t * M)] = ep(r.v[q][t]);
+                    const uint32_t k = i + static_cast<uint32_t>(t * M);
+                    out_global(c.dst)[PROG::cols > 1 ? k * static_cast<uint32_t>(a.out_k) : k] = ep(r.v[q][t]);
                 }
             }
         }
@@ -502,7 +532,7 @@ __device__ __forceinline__ void write_out(unsigned char *lds, const regs<T, NT, 
     using sh = shape<NT, L, PROG, S>;
     constexpr step_info st = sh::st;
     constexpr int R = sh::R, WNS = st.wns;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = lane<PROG>();
 #pragma unroll
     for (int q = 0; q < sh::NB; ++q) {
         const uint32_t i = tid + static_cast<uint32_t>(q * NT);
@@ -524,7 +554,7 @@ __device__ __forceinline__ void read_in(const unsigned char *lds, regs<T, NT, L,
     using sh = shape<NT, L, PROG, S>;
     constexpr step_info st = sh::st;
     constexpr int R = sh::R, M = sh::M;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = lane<PROG>();
 #pragma unroll
     for (int q = 0; q < sh::NB; ++q) {
         const uint32_t i = tid + static_cast<uint32_t>(q * NT);
@@ -623,16 +653,24 @@ __device__ __forceinline__ uint32_t warm_tables(const sfh_tile_args &a)
 
 } // namespace fx
 
-// Entry: tile g = blockIdx.x is one transform (W = 1), addressed as by the generic kernel.
+// Entry: tile g = blockIdx.x is one transform (W = 1) or W columns of a strided axis
+// (W > 1), addressed as by the generic kernel.
 // WAVES: register budget (waves per SIMD); PAD: the pad layout may be used where linear;
 // SPLIT: LDS exchanges move real and imaginary parts separately (half the LDS per tile);
 // PF: pass twiddles are loaded one step ahead (needs registers for them across a step).
-template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2>
+// With W > 1 the launch has W columns per tile, lanes column-fastest (SFH_WFAST), and each
+// column's NT / W work-items run the program on their column, exchanging through the
+// column's own LDS region.
+template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2, int W = 1>
 __global__ __launch_bounds__(NT) SFH_WAVES_PER_EU(WAVES)
 void fixed_fft(const sfh_tile_args a_byval)
 {
-    using prog = fx::program<R1, R2, MODE, PAD>;
+    using prog = fx::program<R1, R2, MODE, PAD, W>;
     constexpr int L = prog::L;
+    constexpr int NTC = NT / W; // work-items per column
+    static_assert(NTC * W == NT, "work-items split evenly over the columns");
+    static_assert(W == 1 || (MODE & (SFH_M_PAIR | SFH_M_DBUF | SFH_M_MID)) == 0,
+                  "columns per tile: plain and pre/post modes with one LDS buffer");
     static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
                   "second FFT: same length, first radix = the first FFT's last");
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -643,11 +681,25 @@ void fixed_fft(const sfh_tile_args a_byval)
 #endif
     SFH_DYNAMIC_LDS(sfh_smem);
     tile_ctx<T> c;
-    set_tile<T, MODE>(a, c, blockIdx.x, threadIdx.x);
+    uint64_t g = blockIdx.x;
+    if constexpr (W > 1) {
+        // g = 8 r + x runs tile 16 (r / 2) + 2 x + (r mod 2): a bijection of the first
+        // 16-multiple of tiles that gives g and g + 8 the tile pair 2 m, 2 m + 1
+        if ((a.flags & SFH_TILE_XCD_PAIR) && g < (a.ntiles & ~uint64_t(15))) {
+            const uint64_t x = g & 7u, r = g >> 3;
+            g = (((r >> 1) << 3) + x) * 2 + (r & 1u);
+        }
+    }
+    set_tile<T, MODE>(a, c, g, threadIdx.x);
+    unsigned char *lds = sfh_smem;
+    if constexpr (W > 1) {
+        constexpr int elem = SPLIT ? static_cast<int>(sizeof(T)) : static_cast<int>(sizeof(cx<T>));
+        lds += static_cast<uint32_t>(threadIdx.x & (W - 1)) * static_cast<uint32_t>(prog::column_pitch(elem) * elem);
+    }
     const uint32_t warmed = fx::warm_tables(a);
-    fx::regs<T, NT, L, prog, 0> r;
-    const fx::twregs<T, NT, L, prog, 0> w0{}; // the first step reads HBM: no pass twiddles
-    fx::run_from<SPLIT, PF, T, NT, L, MODE, prog, 0>(a, c, sfh_smem, static_cast<const cx<T> *>(a.twtab), r, w0);
+    fx::regs<T, NTC, L, prog, 0> r;
+    const fx::twregs<T, NTC, L, prog, 0> w0{}; // the first step reads HBM: no pass twiddles
+    fx::run_from<SPLIT, PF, T, NTC, L, MODE, prog, 0>(a, c, lds, static_cast<const cx<T> *>(a.twtab), r, w0);
 #if defined(__HIP_DEVICE_COMPILE__)
     asm volatile("" ::"v"(warmed)); // the touch loads stay (their values are not needed)
 #else

@@ -55,6 +55,9 @@ struct sfh_fixed_kernel
     int max_tiles_per_cu;
     int n1, n2;
     int r1[8], r2[8];
+    // columns per tile: 1 (one contiguous transform), or W columns of a strided axis (lanes
+    // column-fastest, SFH_WFAST; launches whose column count is a multiple of W)
+    int W;
 };
 extern "C" const sfh_fixed_kernel *sfh_fixed_kernels(int *count);
 extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_args *args, hipStream_t stream);

@@ -866,6 +866,7 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
 }
 
 static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L);
+static const sfh_fixed_kernel *find_fixed_cols(const plan &p, uint64_t n, uint64_t inner);
 static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k);
 
 // A launch that only a length-specialized kernel runs (no generic variant holds its tile):
@@ -908,6 +909,65 @@ static launch make_fixed_launch(plan &p, const sfh_fixed_kernel &k, const geomet
     a.lds_bytes = k.lds_bytes;
     l.desc = what + " L=" + std::to_string(g.L) + " W=1 tiles=" + std::to_string(a.ntiles) + " fixed="
              + std::to_string(k.nt) + "x" + std::to_string(k.L);
+    return l;
+}
+
+// One launch of a column-tile length-specialized kernel over a strided axis: `outer` lines
+// of n points at stride `inner`, W adjacent columns (of the inner extent) per tile, lanes
+// column-fastest, so each row of a tile is one contiguous run of W points.
+static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, uint64_t outer, uint64_t n, uint64_t inner,
+                                     buf_id src, buf_id dst)
+{
+    geometry g;
+    g.L = static_cast<uint32_t>(n);
+    g.W = static_cast<uint32_t>(k.W);
+    g.wcount = inner;
+    g.nblo = outer;
+    g.in_w = g.out_w = 1;
+    g.in_j = g.out_k = static_cast<int64_t>(inner);
+    g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
+    check_span(g);
+    launch l;
+    l.mode = SFH_MI_PLAIN;
+    l.src = src;
+    l.dst = dst;
+    l.variant = 0; // unused: the fixed kernel runs every execution of this launch
+    l.fixed_only = true;
+    sfh_tile_args &a = l.args;
+    a.L = g.L;
+    a.LP = a.L;
+    a.W = g.W;
+    a.logW = ilog2(g.W);
+    a.tpc = static_cast<uint32_t>(k.nt) / g.W;
+    a.logtpc = ilog2(a.tpc);
+    a.wcount = g.wcount;
+    a.nwt = g.wcount / g.W;
+    a.nblo = g.nblo;
+    a.ntiles = a.nwt * g.nblo;
+    a.in_w = a.out_w = 1;
+    a.in_j = a.out_k = static_cast<int32_t>(inner);
+    a.in_blo = a.out_blo = g.in_blo;
+    a.col_div = make_fastdiv(1);
+    a.col_mod = make_fastdiv(1);
+    a.out_sx = 1.0;
+    a.out_sy = 1.0;
+    a.flags |= SFH_WFAST;
+    // adjacent tiles on one XCD (SYCLFFT_HIP_XCD_PAIR=0: off, for measurements)
+    const char *xp = std::getenv("SYCLFFT_HIP_XCD_PAIR");
+    if (!(xp && std::strcmp(xp, "0") == 0)) {
+        a.flags |= SFH_TILE_XCD_PAIR;
+    }
+    a.unit_dim = SFH_UNIT_BLO;
+    l.unit_dim = SFH_UNIT_BLO;
+    l.unit_extent = g.nblo;
+    l.unit_in_stride = g.in_blo;
+    l.unit_out_stride = g.out_blo;
+    l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
+    l.fixed = &k;
+    l.fx_tw_off = fixed_tables(p, k);
+    a.lds_bytes = k.lds_bytes;
+    l.desc = "single L=" + std::to_string(g.L) + " W=" + std::to_string(g.W) + " tiles=" + std::to_string(a.ntiles)
+             + " fixed=" + std::to_string(k.nt) + "x" + std::to_string(k.L) + " cols";
     return l;
 }
 
@@ -1011,6 +1071,13 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
                 g.wcount = outer;
                 g.in_w = g.out_w = static_cast<int64_t>(n);
                 p.launches.push_back(make_fixed_launch(p, *k, g, src, dst, "single"));
+                return;
+            }
+        }
+        // ---------------- one launch of a column-tile length-specialized kernel (strided axis)
+        if (inner > 1 && !p.tune.split) {
+            if (const sfh_fixed_kernel *k = find_fixed_cols(p, n, inner)) {
+                p.launches.push_back(make_fixed_cols_launch(p, *k, outer, n, inner, src, dst));
                 return;
             }
         }
@@ -1749,12 +1816,43 @@ static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_
         skip = std::atoi(e);
     }
     for (int i = 0; i < nk; ++i) {
-        if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L
+        if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L && ks[i].W == 1
             && ks[i].lds_bytes <= k_lds_limit && ks[i].max_tiles_per_cu == 0) {
             if (skip-- > 0) {
                 continue;
             }
             return &ks[i];
+        }
+    }
+    return nullptr;
+}
+
+// A column-tile length-specialized kernel for a strided axis of n points whose inner extent
+// is a multiple of its column count (null: none; none with SYCLFFT_HIP_FIXED=0).
+static const sfh_fixed_kernel *find_fixed_cols(const plan &p, uint64_t n, uint64_t inner)
+{
+    if (const char *fe = std::getenv("SYCLFFT_HIP_FIXED")) {
+        if (std::strcmp(fe, "0") == 0) {
+            return nullptr;
+        }
+    }
+    if (p.cfg.precision != SYCLFFT_HIP_PRECISION_VALU) {
+        return nullptr;
+    }
+    int nk = 0;
+    const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+    int skip = 0;
+    if (const char *e = std::getenv("SYCLFFT_HIP_FIXED_SKIP")) {
+        skip = std::atoi(e);
+    }
+    for (int i = 0; i < nk; ++i) {
+        const sfh_fixed_kernel &k = ks[i];
+        if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == SFH_MI_PLAIN && k.W > 1 && k.L == n
+            && inner % static_cast<uint64_t>(k.W) == 0 && k.lds_bytes <= k_lds_limit && k.max_tiles_per_cu == 0) {
+            if (skip-- > 0) {
+                continue;
+            }
+            return &k;
         }
     }
     return nullptr;
@@ -1795,7 +1893,8 @@ static void choose_fixed(plan &p)
         }
         for (int i = 0; i < nk; ++i) {
             const sfh_fixed_kernel &k = ks[i];
-            if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L && k.lds_bytes <= k_lds_limit
+            if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L && k.W == 1
+                && k.lds_bytes <= k_lds_limit
                 && (k.max_tiles_per_cu == 0
                     || (cus > 0 && a.ntiles <= cus * static_cast<uint64_t>(k.max_tiles_per_cu)))) {
                 if (skip-- > 0) {
@@ -1830,7 +1929,8 @@ static void unfix(launch &l)
 static hipError_t launch_tile(const plan &p, const launch &l, sfh_tile_args &a, uint32_t grid, hipStream_t stream)
 {
     if (l.fixed) {
-        if (a.W != 1 || a.in_j != 1 || a.out_k != 1 || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0) {
+        const uint32_t kw = static_cast<uint32_t>(l.fixed->W);
+        if (a.W != kw || (kw == 1 && (a.in_j != 1 || a.out_k != 1)) || (a.flags & (SFH_LOAD_LDS | SFH_STORE_LDS)) != 0) {
             return hipErrorInvalidConfiguration; // outside the specialized kernel's contract
         }
         a.twtab = l.fx_twtab;

@@ -95,7 +95,15 @@ def test_emulated_measure_mode_writes_cache(tmp_path):
 
 MANY = ["f:4096:3:-1:0:0", "f:300:5:1:3:0", "d:2520:3:1:3:1", "f:10007:3:-1:3:0", "f:32768:3:-1:0:0",
         "d:1009:3:-1:3:1", "f:13x256:3:-1:0:0", "d:256x13:3:1:0:0", "f:32x32x32:3:1:0:1",
-        "d:64x48x10:3:-1:1:0", "f:16384:1:1:1:1"]
+        "d:64x48x10:3:-1:1:0", "f:16384:1:1:1:1", "d:4096x8:2:1:0:1"]
+
+# column-tile length-specialized kernel (f64 4096-point strided axes, 4 columns per tile):
+# 2-D and 3-D, in and out of place, every normalization
+COLS = ["d:4096x4:2:-1:0:0", "d:4096x8:1:1:3:1", "d:4096x12:1:-1:1:0", "d:4096x2x4:1:1:2:0", "d:4096x4:3:1:3:0"]
+
+
+def test_emulated_column_tiles():
+    _run(COLS)
 
 
 @pytest.mark.parametrize("mode", ["1", "2", "3"])

@@ -288,7 +288,9 @@ def test_bluestein(n, double):
     ((16, 16), 3, False), ((32, 32, 32), 2, False), ((64, 48, 10), 1, True),
     ((128, 128), 4, True), ((7, 9, 11), 2, False), ((13, 256), 2, False),
     ((256, 13), 2, True), ((1, 64), 2, False), ((64, 1, 8), 1, True),
-    ((2048, 96), 1, False), ((1024, 1024), 1, False), ((4100, 6), 1, True)])
+    ((2048, 96), 1, False), ((1024, 1024), 1, False), ((4100, 6), 1, True),
+    # f64 4096-point strided axes on the column-tile kernel (4 columns per tile)
+    ((4096, 8), 2, True), ((4096, 4, 3), 1, True), ((4096, 12), 1, True)])
 def test_multid(shape, batch, double):
     n = int(np.prod(shape))
     x = rnd(n * batch, n)
