@@ -671,6 +671,7 @@ void fixed_fft(const sfh_tile_args a_byval)
     static_assert(NTC * W == NT, "work-items split evenly over the columns");
     static_assert(W == 1 || (MODE & (SFH_M_PAIR | SFH_M_DBUF | SFH_M_MID)) == 0,
                   "columns per tile: plain and pre/post modes with one LDS buffer");
+    static_assert(!(MODE & SFH_M_T4), "no four-step twiddle in the length-specialized kernels");
     static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
                   "second FFT: same length, first radix = the first FFT's last");
 #if defined(__HIP_DEVICE_COMPILE__)

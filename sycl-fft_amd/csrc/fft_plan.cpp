@@ -866,7 +866,7 @@ static launch make_launch(plan &p, int mode, geometry g, buf_id src, buf_id dst,
 }
 
 static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L);
-static const sfh_fixed_kernel *find_fixed_cols(const plan &p, uint64_t n, uint64_t inner);
+static const sfh_fixed_kernel *find_fixed_cols(const plan &p, int mode_index, uint64_t n, uint64_t wcount);
 static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k);
 
 // A launch that only a length-specialized kernel runs (no generic variant holds its tile):
@@ -912,20 +912,14 @@ static launch make_fixed_launch(plan &p, const sfh_fixed_kernel &k, const geomet
     return l;
 }
 
-// One launch of a column-tile length-specialized kernel over a strided axis: `outer` lines
-// of n points at stride `inner`, W adjacent columns (of the inner extent) per tile, lanes
-// column-fastest, so each row of a tile is one contiguous run of W points.
-static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, uint64_t outer, uint64_t n, uint64_t inner,
-                                     buf_id src, buf_id dst)
+// One launch of a column-tile length-specialized kernel: W adjacent columns (of g.wcount)
+// per tile, lanes column-fastest, each column's points at stride g.in_j / g.out_k, rows of a
+// tile at g.in_w / g.out_w (plain mode).
+static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, geometry g, buf_id src, buf_id dst,
+                                     const std::string &what)
 {
-    geometry g;
-    g.L = static_cast<uint32_t>(n);
+    g.L = k.L;
     g.W = static_cast<uint32_t>(k.W);
-    g.wcount = inner;
-    g.nblo = outer;
-    g.in_w = g.out_w = 1;
-    g.in_j = g.out_k = static_cast<int64_t>(inner);
-    g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
     check_span(g);
     launch l;
     l.mode = SFH_MI_PLAIN;
@@ -943,12 +937,18 @@ static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, uint64_
     a.wcount = g.wcount;
     a.nwt = g.wcount / g.W;
     a.nblo = g.nblo;
-    a.ntiles = a.nwt * g.nblo;
-    a.in_w = a.out_w = 1;
-    a.in_j = a.out_k = static_cast<int32_t>(inner);
-    a.in_blo = a.out_blo = g.in_blo;
-    a.col_div = make_fastdiv(1);
-    a.col_mod = make_fastdiv(1);
+    a.ntiles = a.nwt * g.nblo * g.nbhi;
+    a.in_w = static_cast<int32_t>(g.in_w);
+    a.in_j = static_cast<int32_t>(g.in_j);
+    a.out_w = static_cast<int32_t>(g.out_w);
+    a.out_k = static_cast<int32_t>(g.out_k);
+    a.in_blo = g.in_blo;
+    a.in_bhi = g.in_bhi;
+    a.out_blo = g.out_blo;
+    a.out_bhi = g.out_bhi;
+    a.col_div = make_fastdiv(static_cast<uint32_t>(g.col_div));
+    a.col_mod = make_fastdiv(static_cast<uint32_t>(g.col_mod));
+    a.col_bmul = static_cast<uint32_t>(g.col_bmul);
     a.out_sx = 1.0;
     a.out_sy = 1.0;
     a.flags |= SFH_WFAST;
@@ -957,16 +957,22 @@ static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, uint64_
     if (!(xp && std::strcmp(xp, "0") == 0)) {
         a.flags |= SFH_TILE_XCD_PAIR;
     }
-    a.unit_dim = SFH_UNIT_BLO;
-    l.unit_dim = SFH_UNIT_BLO;
-    l.unit_extent = g.nblo;
-    l.unit_in_stride = g.in_blo;
-    l.unit_out_stride = g.out_blo;
+    l.unit_dim = g.unit_dim;
+    if (g.unit_dim == SFH_UNIT_BLO) {
+        l.unit_extent = g.nblo;
+        l.unit_in_stride = g.in_blo;
+        l.unit_out_stride = g.out_blo;
+    } else if (g.unit_dim == SFH_UNIT_BHI) {
+        l.unit_extent = g.nbhi;
+        l.unit_in_stride = g.in_bhi;
+        l.unit_out_stride = g.out_bhi;
+    }
+    a.unit_dim = g.unit_dim;
     l.grid = static_cast<uint32_t>(std::min<uint64_t>(a.ntiles, 0xffffffffu));
     l.fixed = &k;
     l.fx_tw_off = fixed_tables(p, k);
     a.lds_bytes = k.lds_bytes;
-    l.desc = "single L=" + std::to_string(g.L) + " W=" + std::to_string(g.W) + " tiles=" + std::to_string(a.ntiles)
+    l.desc = what + " L=" + std::to_string(g.L) + " W=" + std::to_string(g.W) + " tiles=" + std::to_string(a.ntiles)
              + " fixed=" + std::to_string(k.nt) + "x" + std::to_string(k.L) + " cols";
     return l;
 }
@@ -1076,8 +1082,15 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
         }
         // ---------------- one launch of a column-tile length-specialized kernel (strided axis)
         if (inner > 1 && !p.tune.split) {
-            if (const sfh_fixed_kernel *k = find_fixed_cols(p, n, inner)) {
-                p.launches.push_back(make_fixed_cols_launch(p, *k, outer, n, inner, src, dst));
+            if (const sfh_fixed_kernel *k = find_fixed_cols(p, SFH_MI_PLAIN, n, inner)) {
+                geometry g;
+                g.wcount = inner;
+                g.nblo = outer;
+                g.unit_dim = SFH_UNIT_BLO;
+                g.in_w = g.out_w = 1;
+                g.in_j = g.out_k = static_cast<int64_t>(inner);
+                g.in_blo = g.out_blo = static_cast<int64_t>(n * inner);
+                p.launches.push_back(make_fixed_cols_launch(p, *k, g, src, dst, "single"));
                 return;
             }
         }
@@ -1827,9 +1840,10 @@ static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_
     return nullptr;
 }
 
-// A column-tile length-specialized kernel for a strided axis of n points whose inner extent
-// is a multiple of its column count (null: none; none with SYCLFFT_HIP_FIXED=0).
-static const sfh_fixed_kernel *find_fixed_cols(const plan &p, uint64_t n, uint64_t inner)
+// A column-tile length-specialized kernel of `mode_index` for columns of n points whose
+// column count is a multiple of its columns per tile (null: none; none with
+// SYCLFFT_HIP_FIXED=0).
+static const sfh_fixed_kernel *find_fixed_cols(const plan &p, int mode_index, uint64_t n, uint64_t wcount)
 {
     if (const char *fe = std::getenv("SYCLFFT_HIP_FIXED")) {
         if (std::strcmp(fe, "0") == 0) {
@@ -1847,8 +1861,8 @@ static const sfh_fixed_kernel *find_fixed_cols(const plan &p, uint64_t n, uint64
     }
     for (int i = 0; i < nk; ++i) {
         const sfh_fixed_kernel &k = ks[i];
-        if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == SFH_MI_PLAIN && k.W > 1 && k.L == n
-            && inner % static_cast<uint64_t>(k.W) == 0 && k.lds_bytes <= k_lds_limit && k.max_tiles_per_cu == 0) {
+        if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == mode_index && k.W > 1 && k.L == n
+            && wcount % static_cast<uint64_t>(k.W) == 0 && k.lds_bytes <= k_lds_limit && k.max_tiles_per_cu == 0) {
             if (skip-- > 0) {
                 continue;
             }
