@@ -1081,7 +1081,8 @@ static void plan_axis(plan &p, uint64_t outer, uint64_t n, uint64_t inner, buf_i
             }
         }
         // ---------------- one launch of a column-tile length-specialized kernel (strided axis)
-        if (inner > 1 && !p.tune.split) {
+        // (tile offsets are 32-bit: the axis' span must stay below 2^31 elements)
+        if (inner > 1 && !p.tune.split && inner * (n - 1) + 64 < (uint64_t(1) << 31)) {
             if (const sfh_fixed_kernel *k = find_fixed_cols(p, SFH_MI_PLAIN, n, inner)) {
                 geometry g;
                 g.wcount = inner;
