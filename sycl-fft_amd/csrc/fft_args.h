@@ -53,7 +53,11 @@ enum sfh_mode : int
     SFH_M_PAIR = 64,
     // length-specialized kernels: two LDS buffers used in turn by successive exchanges, so an
     // exchange needs one barrier (between its writes and reads) instead of two
-    SFH_M_DBUF = 128
+    SFH_M_DBUF = 128,
+    // length-specialized kernels, C2R tangle on load (real plans): point j of the length-L
+    // complex FFT is Z[j] = (X[j] + conj X[L-j]) + i W_2L^-j (X[j] - conj X[L-j]) of the
+    // L + 1-point half spectrum X at the tile's source, with pre[j] = W_2L^j
+    SFH_M_C2R = 256
 };
 
 // Run-time flags.

@@ -18,7 +18,10 @@ sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
     k.fn = reinterpret_cast<const void *>(&sfh::fixed_fft<T, NT, WAVES, MODE, PAD, SPLIT, PF, R1, R2, W>);
     k.W = W;
     k.is_double = sizeof(T) == 8;
-    k.mode_index = (MODE & SFH_M_PAIR) ? SFH_FX_BLUE_PAIR : ((MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED : SFH_MI_PLAIN);
+    k.mode_index = (MODE & SFH_M_PAIR)  ? SFH_FX_BLUE_PAIR
+                   : (MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED
+                   : (MODE & SFH_M_C2R) ? SFH_FX_C2R
+                                        : SFH_MI_PLAIN;
     k.nt = NT;
     k.L = static_cast<uint32_t>(prog::L);
     // LDS bytes per tile: one complex value per slot, or one real with the split exchange
@@ -102,6 +105,16 @@ const sfh_fixed_kernel *table(int *count)
         // points or 64 x 32 points tied the generic kernel, 199-222 against 201-217 us.)
         entry<double, 256, 4, 0, true, true, false, radices<16, 16, 16>, none>(),
         entry<double, 128, 4, 0, true, true, false, radices<16, 16, 8>, none>(),
+        // complex<float> 2048 (tied the generic kernel as a plain entry; present for its C2R form)
+        entry<float, 64, 4, 0, true, true, false, radices<32, 16, 4>, none>(),
+        // C2R real plans: the tangle of the half spectrum fused into the first pass of the
+        // length-M backward FFT (one HBM pass instead of two); same radix programs as the plain
+        // entries of these lengths, whose pass tables they share
+        entry<double, 128, 4, SFH_M_C2R, true, true, false, radices<16, 16, 8>, none>(),
+        entry<double, 256, 4, SFH_M_C2R, true, true, false, radices<16, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_C2R, true, true, false, radices<32, 16, 4>, none>(),
+        entry<float, 128, 4, SFH_M_C2R, true, true, false, radices<32, 16, 8>, none>(),
+        entry<float, 256, 2, SFH_M_C2R, true, true, false, radices<32, 16, 16>, none>(),
         // column tiles (strided axes): complex<double> 4096 x 4 columns, 1024 work-items x 16
         // points (256 per column), split regions of 33 KiB per column
         entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(),

@@ -217,6 +217,21 @@ template <typename T, int MODE, int COLS = 1>
 __device__ __forceinline__ cx<T> load_point(const sfh_tile_args &a, const tile_ctx<T> &c, uint32_t j)
 {
     const uint32_t at = COLS > 1 ? j * static_cast<uint32_t>(a.in_j) : j;
+    if constexpr (MODE & SFH_M_C2R) {
+        // Z[j] = (X[j] + conj X[L-j]) + i W^-j (X[j] - conj X[L-j]), W = W_2L
+        const cx<T> x = in_global(c.src)[j];
+        cx<T> m = in_global(c.src)[a.L - j];
+        m.y = -m.y;
+        const cx<T> s = x + m, d = x - m;
+        cx<T> w = static_cast<const cx<T> *>(a.pre)[j];
+        w.y = -w.y;
+        const cx<T> t = cmul(w, d);
+        cx<T> v = {s.x - t.y, s.y + t.x};
+        if (a.flags & SFH_CONJ_IN) {
+            v.y = -v.y;
+        }
+        return v;
+    }
     if constexpr (MODE & SFH_M_PRE) {
         const uint32_t p = j * a.pre_jmul + c.col * a.pre_cmul;
         if (p >= a.nvalid_in) {
@@ -672,6 +687,8 @@ void fixed_fft(const sfh_tile_args a_byval)
     static_assert(W == 1 || (MODE & (SFH_M_PAIR | SFH_M_DBUF | SFH_M_MID)) == 0,
                   "columns per tile: plain and pre/post modes with one LDS buffer");
     static_assert(!(MODE & SFH_M_T4), "no four-step twiddle in the length-specialized kernels");
+    static_assert(!(MODE & SFH_M_C2R) || (W == 1 && (MODE & ~(SFH_M_C2R | SFH_M_DBUF)) == 0),
+                  "C2R tangle: one contiguous transform per tile, plain mode");
     static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
                   "second FFT: same length, first radix = the first FFT's last");
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -41,6 +41,8 @@ struct sfh_kernel_variant
 // mode_index of the length-specialized Bluestein form over two workgroups per transform
 // (no generic counterpart)
 #define SFH_FX_BLUE_PAIR 64
+// mode_index of the length-specialized C2R kernels (tangle fused into the first pass)
+#define SFH_FX_C2R 65
 
 struct sfh_fixed_kernel
 {

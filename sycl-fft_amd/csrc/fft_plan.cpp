@@ -306,6 +306,9 @@ struct plan
     // plus one untangle / tangle pass with the table W_N^k (k = 0..real_m) into / out of scratch
     int real{};
     struct plan *half{};
+    // 1-D C2R plan whose half-length plan is one length-specialized launch: launches[0] runs
+    // the tangle fused into that launch's first pass (one HBM pass)
+    bool real_fused{};
     uint64_t real_m{};
     void *real_tw{};
     void *real_scratch{};
@@ -3147,6 +3150,12 @@ static void execute_real(
         hip_check(static_cast<hipError_t>(sfh_launch_real_pass(0, p.real_scratch, out, p.real_tw, M, B, p.dbl ? 1 : 0,
                                                                p.scale, stream)),
                   "untangle launch");
+    } else if (p.real_fused) {
+        launch &l = p.launches[0];
+        sfh_tile_args a = l.args;
+        a.in = in;
+        a.out = out;
+        hip_check(launch_tile(*p.half, l, a, l.grid, stream), "fused c2r launch");
     } else {
         hip_check(static_cast<hipError_t>(sfh_launch_real_pass(1, in, p.real_scratch, p.real_tw, M, B, p.dbl ? 1 : 0,
                                                                p.scale, stream)),
@@ -3216,6 +3225,34 @@ static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
     hip_check(hipMemcpy(p->real_tw, host.data(), host.size(), hipMemcpyHostToDevice), "hipMemcpy(real twiddles)");
     hip_check(hipMalloc(&p->real_scratch, cfg->batch_count * M * p->esize), "hipMalloc(real scratch)");
     hip_check(hipEventCreateWithFlags(&p->last, hipEventDisableTiming | hipEventDisableSystemFence), "hipEventCreate");
+    // C2R: when the half-length plan is one length-specialized launch and a C2R form of the
+    // same radix program exists, the tangle moves into that launch's loads (SYCLFFT_HIP_FIXED=0
+    // keeps the separate pass, as it keeps the generic kernels)
+    if (p->real == 2 && hp->launches.size() == 1 && !hp->copy_only && hp->launches[0].fixed
+        && hp->launches[0].fixed->W == 1 && hp->launches[0].fixed->mode_index == SFH_MI_PLAIN
+        && hp->launches[0].args.in_w == static_cast<int32_t>(M)) {
+        const sfh_fixed_kernel &pk = *hp->launches[0].fixed;
+        int nk = 0;
+        const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+        for (int i = 0; i < nk; ++i) {
+            const sfh_fixed_kernel &k = ks[i];
+            if (k.mode_index != SFH_FX_C2R || k.is_double != pk.is_double || k.L != pk.L || k.nt != pk.nt
+                || k.n1 != pk.n1 || !std::equal(pk.r1, pk.r1 + pk.n1, k.r1)) {
+                continue;
+            }
+            launch l = hp->launches[0]; // pass tables, epilogue (conjugation) of the backward plan
+            l.fixed = &k;
+            l.args.pre = p->real_tw;
+            l.args.in_w = static_cast<int32_t>(M + 1);
+            l.args.lds_bytes = k.lds_bytes;
+            l.args.out_sx *= p->scale;
+            l.args.out_sy *= p->scale;
+            l.desc = "c2r-fused m=" + std::to_string(M) + " " + l.desc;
+            p->launches.push_back(l);
+            p->real_fused = true;
+            break;
+        }
+    }
     p->half = hp.release();
     trace(std::string(p->real == 1 ? "real r2c" : "real c2r") + " n=" + std::to_string(N) + " over complex n=" +
           std::to_string(M));
@@ -3506,6 +3543,9 @@ static std::string describe_plan(const plan &p)
         const std::string a = describe_plan(*p.half), b = describe_plan(*p.core);
         const std::string a0 = a.substr(0, a.rfind("tuning")), b0 = b.substr(0, b.rfind("tuning"));
         return (p.real == 1 ? a0 + b0 : b0 + a0) + "tuning " + p.half->tune.str() + "\n";
+    }
+    if (p.real && p.real_fused) {
+        return p.launches[0].desc + "\ntuning " + p.half->tune.str() + "\n";
     }
     if (p.real) {
         const std::string pass = p.real == 1 ? "r2c-untangle" : "c2r-tangle";
@@ -3932,7 +3972,7 @@ extern "C" uint32_t syclfft_hip_plan_launch_count(const void *plan)
         return syclfft_hip_plan_launch_count(p->half) + syclfft_hip_plan_launch_count(p->core);
     }
     if (p->real) {
-        return 1u + syclfft_hip_plan_launch_count(p->half);
+        return p->real_fused ? 1u : 1u + syclfft_hip_plan_launch_count(p->half);
     }
     if (p->layout) {
         return (p->lay_fused ? 0u : 2u) + syclfft_hip_plan_launch_count(p->core);
