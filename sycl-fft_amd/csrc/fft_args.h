@@ -57,7 +57,12 @@ enum sfh_mode : int
     // length-specialized kernels, C2R tangle on load (real plans): point j of the length-L
     // complex FFT is Z[j] = (X[j] + conj X[L-j]) + i W_2L^-j (X[j] - conj X[L-j]) of the
     // L + 1-point half spectrum X at the tile's source, with pre[j] = W_2L^j
-    SFH_M_C2R = 256
+    SFH_M_C2R = 256,
+    // length-specialized kernels, R2C untangle on store (real plans): the length-L FFT Z of
+    // the real input read as L complex values goes through LDS, and output k = 0..L of the
+    // tile's destination is X[k] = 1/2 (Z[k] + conj Z[L-k]) - i/2 W_2L^k (Z[k] - conj Z[L-k])
+    // (Z[L] = Z[0]), with pre[k] = W_2L^k
+    SFH_M_R2C = 512
 };
 
 // Run-time flags.

@@ -21,6 +21,7 @@ sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
     k.mode_index = (MODE & SFH_M_PAIR)  ? SFH_FX_BLUE_PAIR
                    : (MODE & SFH_M_MID) ? SFH_MI_BLUE_FUSED
                    : (MODE & SFH_M_C2R) ? SFH_FX_C2R
+                   : (MODE & SFH_M_R2C) ? SFH_FX_R2C
                                         : SFH_MI_PLAIN;
     k.nt = NT;
     k.L = static_cast<uint32_t>(prog::L);
@@ -115,6 +116,13 @@ const sfh_fixed_kernel *table(int *count)
         entry<float, 64, 4, SFH_M_C2R, true, true, false, radices<32, 16, 4>, none>(),
         entry<float, 128, 4, SFH_M_C2R, true, true, false, radices<32, 16, 8>, none>(),
         entry<float, 256, 2, SFH_M_C2R, true, true, false, radices<32, 16, 16>, none>(),
+        // R2C real plans: the untangle fused into the last pass (one more LDS exchange, whole
+        // complex values), same radix programs as the plain entries
+        entry<double, 128, 4, SFH_M_R2C, true, false, false, radices<16, 16, 8>, none>(),
+        entry<double, 256, 4, SFH_M_R2C, true, false, false, radices<16, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<32, 16, 4>, none>(),
+        entry<float, 128, 4, SFH_M_R2C, true, false, false, radices<32, 16, 8>, none>(),
+        entry<float, 256, 2, SFH_M_R2C, true, false, false, radices<32, 16, 16>, none>(),
         // column tiles (strided axes): complex<double> 4096 x 4 columns, 1024 work-items x 16
         // points (256 per column), split regions of 33 KiB per column
         entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(),

@@ -517,6 +517,44 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
     }
 }
 
+// R2C untangle (SFH_M_R2C): the last step's outputs Z[k] go to LDS in natural order (whole
+// complex values: R2C kernels exchange without the split), then work-item j computes outputs
+// k = j, j + NT, ... <= L from Z[k] and Z[L - k]:
+//   X[k] = 1/2 (Z[k] + conj Z[L-k]) - i/2 W^k (Z[k] - conj Z[L-k]),  W = W_2L, Z[L] = Z[0],
+// and stores them through the epilogue (scale) to the tile's destination (L + 1 points).
+template <typename T, int NT, int L, int MODE, class PROG, int S>
+__device__ __forceinline__ void store_r2c(const sfh_tile_args &a, const tile_ctx<T> &c, unsigned char *lds,
+                                          const regs<T, NT, L, PROG, S> &r)
+{
+    using sh = shape<NT, L, PROG, S>;
+    constexpr int R = sh::R, M = sh::M;
+    const uint32_t tid = lane<PROG>();
+    cx<T> *z = reinterpret_cast<cx<T> *>(lds);
+    __syncthreads(); // the last exchange's LDS reads are done
+#pragma unroll
+    for (int q = 0; q < sh::NB; ++q) {
+        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
+        if (sh::has(q, i)) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                z[i + static_cast<uint32_t>(t * M)] = r.v[q][t];
+            }
+        }
+    }
+    __syncthreads();
+    const epilogue<T> ep(a);
+    const cx<T> *w = static_cast<const cx<T> *>(a.pre);
+    for (uint32_t k = tid; k <= static_cast<uint32_t>(L); k += static_cast<uint32_t>(NT)) {
+        const cx<T> zk = z[k == static_cast<uint32_t>(L) ? 0u : k];
+        cx<T> zm = z[k == 0u ? 0u : static_cast<uint32_t>(L) - k];
+        zm.y = -zm.y;
+        const cx<T> s = zk + zm, d = zk - zm;
+        const cx<T> t = cmul(w[k], d);
+        // 1/2 (s - i t) = 1/2 (s.x + t.y, s.y - t.x)
+        out_global(c.dst)[k] = ep(cx<T>{T(0.5) * (s.x + t.y), T(0.5) * (s.y - t.x)});
+    }
+}
+
 // Component access of one point: the whole complex value (PART = 2), or its real (0) or
 // imaginary (1) part alone for the split exchange.
 template <int PART, typename T, typename U>
@@ -629,6 +667,8 @@ __device__ __forceinline__ void run_from(const sfh_tile_args &a, const tile_ctx<
         compute<PF, T, NT, L, MODE, PROG, S>(a, c, tw, w, r);
         if constexpr ((MODE & SFH_M_PAIR) != 0) {
             store_pair<T, NT, L, MODE, PROG, S>(a, c, lds, r);
+        } else if constexpr ((MODE & SFH_M_R2C) != 0) {
+            store_r2c<T, NT, L, MODE, PROG, S>(a, c, lds, r);
         } else {
             store<T, NT, L, MODE, PROG, S>(a, c, r);
         }
@@ -689,6 +729,8 @@ void fixed_fft(const sfh_tile_args a_byval)
     static_assert(!(MODE & SFH_M_T4), "no four-step twiddle in the length-specialized kernels");
     static_assert(!(MODE & SFH_M_C2R) || (W == 1 && (MODE & ~(SFH_M_C2R | SFH_M_DBUF)) == 0),
                   "C2R tangle: one contiguous transform per tile, plain mode");
+    static_assert(!(MODE & SFH_M_R2C) || (W == 1 && !SPLIT && (MODE & ~(SFH_M_R2C | SFH_M_DBUF)) == 0),
+                  "R2C untangle: one contiguous transform per tile, whole-value exchanges, plain mode");
     static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
                   "second FFT: same length, first radix = the first FFT's last");
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -43,6 +43,8 @@ struct sfh_kernel_variant
 #define SFH_FX_BLUE_PAIR 64
 // mode_index of the length-specialized C2R kernels (tangle fused into the first pass)
 #define SFH_FX_C2R 65
+// mode_index of the length-specialized R2C kernels (untangle fused into the last pass)
+#define SFH_FX_R2C 66
 
 struct sfh_fixed_kernel
 {

@@ -3145,17 +3145,17 @@ static void execute_real(
             execute_checked(*p.core, in, p.real_scratch, stream, nullptr, 0, nullptr, p.core->scratch);
             execute_plan(*p.half, p.real_scratch, out, stream, nullptr, 0, nullptr);
         }
-    } else if (p.real == 1) {
-        execute_plan(*p.half, in, p.real_scratch, stream, nullptr, 0, nullptr);
-        hip_check(static_cast<hipError_t>(sfh_launch_real_pass(0, p.real_scratch, out, p.real_tw, M, B, p.dbl ? 1 : 0,
-                                                               p.scale, stream)),
-                  "untangle launch");
     } else if (p.real_fused) {
         launch &l = p.launches[0];
         sfh_tile_args a = l.args;
         a.in = in;
         a.out = out;
-        hip_check(launch_tile(*p.half, l, a, l.grid, stream), "fused c2r launch");
+        hip_check(launch_tile(*p.half, l, a, l.grid, stream), p.real == 1 ? "fused r2c launch" : "fused c2r launch");
+    } else if (p.real == 1) {
+        execute_plan(*p.half, in, p.real_scratch, stream, nullptr, 0, nullptr);
+        hip_check(static_cast<hipError_t>(sfh_launch_real_pass(0, p.real_scratch, out, p.real_tw, M, B, p.dbl ? 1 : 0,
+                                                               p.scale, stream)),
+                  "untangle launch");
     } else {
         hip_check(static_cast<hipError_t>(sfh_launch_real_pass(1, in, p.real_scratch, p.real_tw, M, B, p.dbl ? 1 : 0,
                                                                p.scale, stream)),
@@ -3248,6 +3248,33 @@ static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
             l.args.out_sx *= p->scale;
             l.args.out_sy *= p->scale;
             l.desc = "c2r-fused m=" + std::to_string(M) + " " + l.desc;
+            p->launches.push_back(l);
+            p->real_fused = true;
+            break;
+        }
+    }
+    // R2C: the same with the untangle in that launch's stores (L + 1 outputs per transform)
+    if (p->real == 1 && hp->launches.size() == 1 && !hp->copy_only && hp->launches[0].fixed
+        && hp->launches[0].fixed->W == 1 && hp->launches[0].fixed->mode_index == SFH_MI_PLAIN
+        && hp->launches[0].args.in_w == static_cast<int32_t>(M)
+        && hp->launches[0].args.out_w == static_cast<int32_t>(M)) {
+        const sfh_fixed_kernel &pk = *hp->launches[0].fixed;
+        int nk = 0;
+        const sfh_fixed_kernel *ks = sfh_fixed_kernels(&nk);
+        for (int i = 0; i < nk; ++i) {
+            const sfh_fixed_kernel &k = ks[i];
+            if (k.mode_index != SFH_FX_R2C || k.is_double != pk.is_double || k.L != pk.L || k.nt != pk.nt
+                || k.n1 != pk.n1 || !std::equal(pk.r1, pk.r1 + pk.n1, k.r1)) {
+                continue;
+            }
+            launch l = hp->launches[0];
+            l.fixed = &k;
+            l.args.pre = p->real_tw;
+            l.args.out_w = static_cast<int32_t>(M + 1);
+            l.args.lds_bytes = k.lds_bytes;
+            l.args.out_sx *= p->scale;
+            l.args.out_sy *= p->scale;
+            l.desc = "r2c-fused m=" + std::to_string(M) + " " + l.desc;
             p->launches.push_back(l);
             p->real_fused = true;
             break;

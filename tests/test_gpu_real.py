@@ -29,8 +29,8 @@ def tol(n, double):
     return 4 * max(1.0, np.log2(n)) * U[double]
 
 
-CASES = [(2, 5), (16, 7), (30, 4), (256, 16), (1000, 3), (4096, 8), (16384, 2), (20014, 2), (65536, 2),
-         (1 << 20, 1), (2520, 64)]
+CASES = [(2, 5), (16, 7), (30, 4), (256, 16), (1000, 3), (4096, 8), (8192, 300), (16384, 2), (20014, 2),
+         (65536, 2), (1 << 20, 1), (2520, 64)]
 
 
 @pytest.mark.parametrize("double", [False, True])
