@@ -11,7 +11,7 @@ namespace
 using sfh::fx::radices;
 
 template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2, int W = 1>
-sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
+sfh_fixed_kernel entry(int max_tiles_per_cu = 0, int half_only = 0)
 {
     using prog = sfh::fx::program<R1, R2, MODE, PAD, W>;
     sfh_fixed_kernel k{};
@@ -31,6 +31,7 @@ sfh_fixed_kernel entry(int max_tiles_per_cu = 0)
     k.lds_bytes = W > 1 ? static_cast<uint32_t>(W * prog::column_pitch(elem) * elem)
                         : static_cast<uint32_t>(prog::slots() * elem * ((MODE & SFH_M_DBUF) ? 2 : 1));
     k.max_tiles_per_cu = max_tiles_per_cu;
+    k.half_only = half_only;
     k.n1 = R1::n;
     for (int i = 0; i < R1::n; ++i) {
         k.r1[i] = R1::v[i];
@@ -108,6 +109,10 @@ const sfh_fixed_kernel *table(int *count)
         entry<double, 128, 4, 0, true, true, false, radices<16, 16, 8>, none>(),
         // complex<float> 2048 (tied the generic kernel as a plain entry; present for its C2R form)
         entry<float, 64, 4, 0, true, true, false, radices<32, 16, 4>, none>(),
+        // complex<float> 1024: one wave x 16 points, for real plans of N = 2048 only (at 512 MiB
+        // of complex values 210-218 us against 204-215 us on the generic kernel; its C2R / R2C
+        // forms remove a pass)
+        entry<float, 64, 4, 0, true, true, false, radices<16, 16, 4>, none>(0, 1),
         // C2R real plans: the tangle of the half spectrum fused into the first pass of the
         // length-M backward FFT (one HBM pass instead of two); same radix programs as the plain
         // entries of these lengths, whose pass tables they share
@@ -116,6 +121,7 @@ const sfh_fixed_kernel *table(int *count)
         entry<float, 64, 4, SFH_M_C2R, true, true, false, radices<32, 16, 4>, none>(),
         entry<float, 128, 4, SFH_M_C2R, true, true, false, radices<32, 16, 8>, none>(),
         entry<float, 256, 2, SFH_M_C2R, true, true, false, radices<32, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_C2R, true, true, false, radices<16, 16, 4>, none>(),
         // R2C real plans: the untangle fused into the last pass (one more LDS exchange, whole
         // complex values), same radix programs as the plain entries
         entry<double, 128, 4, SFH_M_R2C, true, false, false, radices<16, 16, 8>, none>(),
@@ -123,9 +129,13 @@ const sfh_fixed_kernel *table(int *count)
         entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<32, 16, 4>, none>(),
         entry<float, 128, 4, SFH_M_R2C, true, false, false, radices<32, 16, 8>, none>(),
         entry<float, 256, 2, SFH_M_R2C, true, false, false, radices<32, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<16, 16, 4>, none>(),
         // column tiles (strided axes): complex<double> 4096 x 4 columns, 1024 work-items x 16
         // points (256 per column), split regions of 33 KiB per column
         entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(),
+        // complex<float> 2048 x 8 columns (rows of 64 bytes, as above), 1024 work-items x 16
+        // points, split regions of 8.3 KiB per column
+        entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 8>, none, 8>(),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

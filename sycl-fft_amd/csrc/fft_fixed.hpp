@@ -217,6 +217,11 @@ template <typename T, int MODE, int COLS = 1>
 __device__ __forceinline__ cx<T> load_point(const sfh_tile_args &a, const tile_ctx<T> &c, uint32_t j)
 {
     const uint32_t at = COLS > 1 ? j * static_cast<uint32_t>(a.in_j) : j;
+    if constexpr (COLS > 1) {
+        if (!c.wvalid) { // a column past the last of a ragged tile: no access
+            return {T(0), T(0)};
+        }
+    }
     if constexpr (MODE & SFH_M_C2R) {
         // Z[j] = (X[j] + conj X[L-j]) + i W^-j (X[j] - conj X[L-j]), W = W_2L
         const cx<T> x = in_global(c.src)[j];
@@ -506,7 +511,7 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
 #pragma unroll
         for (int q = 0; q < sh::NB; ++q) {
             const uint32_t i = tid + static_cast<uint32_t>(q * NT);
-            if (sh::has(q, i)) {
+            if (sh::has(q, i) && (PROG::cols == 1 || c.wvalid)) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     const uint32_t k = i + static_cast<uint32_t>(t * M);

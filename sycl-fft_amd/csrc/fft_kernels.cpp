@@ -166,7 +166,8 @@ extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_a
     const uint32_t kw = static_cast<uint32_t>(kernel->W);
     if (args->ntiles > 0xffffffffull / static_cast<uint64_t>(kernel->nt) || args->W != kw || args->L != kernel->L
         || args->lds_bytes > 163840
-        || (kw > 1 && ((args->flags & SFH_WFAST) == 0 || (1u << args->logW) != kw || args->wcount % kw != 0))) {
+        || (kw > 1 && ((args->flags & SFH_WFAST) == 0 || (1u << args->logW) != kw
+                        || args->nwt != (args->wcount + kw - 1) / kw))) {
         return static_cast<int>(hipErrorInvalidConfiguration);
     }
     static std::once_flag lds_once;

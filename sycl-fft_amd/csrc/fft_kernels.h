@@ -50,7 +50,7 @@ struct sfh_fixed_kernel
 {
     const void *fn;
     int is_double;
-    int mode_index; // SFH_MI_PLAIN, SFH_MI_BLUE_FUSED or SFH_FX_BLUE_PAIR
+    int mode_index; // SFH_MI_PLAIN, SFH_MI_BLUE_FUSED, SFH_FX_BLUE_PAIR, SFH_FX_C2R or SFH_FX_R2C
     int nt;
     uint32_t L;
     uint32_t lds_bytes;
@@ -60,8 +60,11 @@ struct sfh_fixed_kernel
     int n1, n2;
     int r1[8], r2[8];
     // columns per tile: 1 (one contiguous transform), or W columns of a strided axis (lanes
-    // column-fastest, SFH_WFAST; launches whose column count is a multiple of W)
+    // column-fastest, SFH_WFAST; the last tile of a row may hold fewer columns)
     int W;
+    // 1: used only for the half-length FFT of a real plan (where its C2R / R2C forms fuse
+    // the tangle or untangle pass), not for complex plans of its length
+    int half_only;
 };
 extern "C" const sfh_fixed_kernel *sfh_fixed_kernels(int *count);
 extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_args *args, hipStream_t stream);

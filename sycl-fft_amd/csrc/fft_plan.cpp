@@ -938,7 +938,7 @@ static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, geometr
     a.tpc = static_cast<uint32_t>(k.nt) / g.W;
     a.logtpc = ilog2(a.tpc);
     a.wcount = g.wcount;
-    a.nwt = g.wcount / g.W;
+    a.nwt = (g.wcount + g.W - 1) / g.W; // the last tile of a row may hold fewer columns
     a.nblo = g.nblo;
     a.ntiles = a.nwt * g.nblo * g.nbhi;
     a.in_w = static_cast<int32_t>(g.in_w);
@@ -1815,6 +1815,10 @@ static int64_t fixed_tables(plan &p, const sfh_fixed_kernel &k)
 
 // A length-specialized kernel of the plan's precision for `mode_index` and length L (null:
 // none within the LDS limit; none at all with SYCLFFT_HIP_FIXED=0).
+// Set while a real plan builds its half-length complex plan: entries marked half_only
+// qualify only then.
+static thread_local bool t_real_half = false;
+
 static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_t L)
 {
     if (const char *fe = std::getenv("SYCLFFT_HIP_FIXED")) {
@@ -1834,7 +1838,7 @@ static const sfh_fixed_kernel *find_fixed(const plan &p, int mode_index, uint64_
     }
     for (int i = 0; i < nk; ++i) {
         if (ks[i].is_double == (p.dbl ? 1 : 0) && ks[i].mode_index == mode_index && ks[i].L == L && ks[i].W == 1
-            && ks[i].lds_bytes <= k_lds_limit && ks[i].max_tiles_per_cu == 0) {
+            && ks[i].lds_bytes <= k_lds_limit && ks[i].max_tiles_per_cu == 0 && (!ks[i].half_only || t_real_half)) {
             if (skip-- > 0) {
                 continue;
             }
@@ -1865,8 +1869,10 @@ static const sfh_fixed_kernel *find_fixed_cols(const plan &p, int mode_index, ui
     }
     for (int i = 0; i < nk; ++i) {
         const sfh_fixed_kernel &k = ks[i];
+        // (a ragged last tile idles W - wcount mod W columns: taken from 8 tiles per row on)
+        const uint64_t kw = static_cast<uint64_t>(k.W);
         if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == mode_index && k.W > 1 && k.L == n
-            && wcount % static_cast<uint64_t>(k.W) == 0 && k.lds_bytes <= k_lds_limit && k.max_tiles_per_cu == 0) {
+            && (wcount % kw == 0 || wcount >= 8 * kw) && k.lds_bytes <= k_lds_limit && k.max_tiles_per_cu == 0) {
             if (skip-- > 0) {
                 continue;
             }
@@ -1912,7 +1918,7 @@ static void choose_fixed(plan &p)
         for (int i = 0; i < nk; ++i) {
             const sfh_fixed_kernel &k = ks[i];
             if (k.is_double == (p.dbl ? 1 : 0) && k.mode_index == l.mode && k.L == a.L && k.W == 1
-                && k.lds_bytes <= k_lds_limit
+                && k.lds_bytes <= k_lds_limit && (!k.half_only || t_real_half)
                 && (k.max_tiles_per_cu == 0
                     || (cus > 0 && a.ntiles <= cus * static_cast<uint64_t>(k.max_tiles_per_cu)))) {
                 if (skip-- > 0) {
@@ -3192,7 +3198,16 @@ static plan *create_real_plan(const syclfft_hip_plan_config_v1 *cfg)
     syclfft_hip_plan_config_v1 h = *cfg;
     h.lengths[0] = cfg->lengths[0] / 2;
     h.normalization = SYCLFFT_HIP_NORM_NONE;
-    std::unique_ptr<plan, void (*)(plan *)> hp(create_plan(&h), release);
+    struct half_scope
+    {
+        half_scope() { t_real_half = true; }
+        ~half_scope() { t_real_half = false; }
+    };
+    std::unique_ptr<plan, void (*)(plan *)> hp(nullptr, release);
+    {
+        const half_scope scope;
+        hp.reset(create_plan(&h));
+    }
     std::unique_ptr<plan, void (*)(plan *)> p(new plan, release);
     p->cfg = *cfg;
     p->device = hp->device;

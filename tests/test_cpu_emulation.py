@@ -99,7 +99,10 @@ MANY = ["f:4096:3:-1:0:0", "f:300:5:1:3:0", "d:2520:3:1:3:1", "f:10007:3:-1:3:0"
 
 # column-tile length-specialized kernel (f64 4096-point strided axes, 4 columns per tile):
 # 2-D and 3-D, in and out of place, every normalization
-COLS = ["d:4096x4:2:-1:0:0", "d:4096x8:1:1:3:1", "d:4096x12:1:-1:1:0", "d:4096x2x4:1:1:2:0", "d:4096x4:3:1:3:0"]
+COLS = ["d:4096x4:2:-1:0:0", "d:4096x8:1:1:3:1", "d:4096x12:1:-1:1:0", "d:4096x2x4:1:1:2:0", "d:4096x4:3:1:3:0",
+        # ragged last tiles (33 and 65 columns), fp32 2048 x 8 columns, multi-D real plans over them
+        "d:4096x33:1:-1:0:0", "c:2048x65:1:-1:1:0", "c:2048x8:2:1:3:1", "R:2048x128:2:-1:0:0",
+        "R:2048x128:2:1:2:0"]
 
 
 def test_emulated_column_tiles():
@@ -192,7 +195,11 @@ def test_emulated_real_plans():
           "R:4096:2:1:2:0", "Q:1000:2:1:3:0", "R:20014:1:-1:0:0", "Q:2:2:1:0:0", "R:65536:1:1:1:0",
           # multi-D: the last axis as 1-D real rows, the leading axes over the half spectrum
           "R:8x6:2:-1:0:0", "Q:8x6:2:1:3:0", "R:12x10x8:1:-1:2:0", "Q:5x14:3:1:1:0", "R:32x64:2:1:0:0",
-          "Q:7x9x10:1:-1:3:0", "R:3x1000:2:1:2:0"])
+          "Q:7x9x10:1:-1:3:0", "R:3x1000:2:1:2:0",
+          # fused: the untangle in the last pass / the tangle in the first pass of the
+          # length-specialized half-length FFT (M = 1024, 2048, 4096, 8192)
+          "R:2048:3:-1:0:0", "R:2048:3:1:1:0", "Q:4096:3:-1:3:0", "Q:4096:2:1:3:0", "R:16384:1:-1:1:0",
+          "Q:8192:2:-1:2:0"])
 
 
 def test_emulated_layout_plans():

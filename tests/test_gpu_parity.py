@@ -290,7 +290,10 @@ def test_bluestein(n, double):
     ((256, 13), 2, True), ((1, 64), 2, False), ((64, 1, 8), 1, True),
     ((2048, 96), 1, False), ((1024, 1024), 1, False), ((4100, 6), 1, True),
     # f64 4096-point strided axes on the column-tile kernel (4 columns per tile)
-    ((4096, 8), 2, True), ((4096, 4, 3), 1, True), ((4096, 12), 1, True)])
+    ((4096, 8), 2, True), ((4096, 4, 3), 1, True), ((4096, 12), 1, True),
+    # column tiles with a ragged last tile per row (33 = 8 x 4 + 1, 65 = 8 x 8 + 1 columns),
+    # and fp32 2048-point strided axes (8 columns per tile)
+    ((4096, 33), 1, True), ((2048, 65), 2, False), ((2048, 8, 9), 1, False), ((2048, 2048), 1, False)])
 def test_multid(shape, batch, double):
     n = int(np.prod(shape))
     x = rnd(n * batch, n)

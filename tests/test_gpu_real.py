@@ -119,7 +119,8 @@ def test_real_plan_buffer_checks():
 
 # multi-D real plans: the last axis as 1-D real rows, then the leading axes of the half spectrum
 MD_CASES = [([8, 6], 2), ([32, 64], 3), ([12, 10, 8], 1), ([5, 14], 3), ([64, 2048], 2), ([256, 256], 2),
-            ([7, 9, 10], 2), ([1000, 30], 1), ([2, 2], 4), ([17, 4096], 1), ([4, 8, 10008], 1)]
+            ([7, 9, 10], 2), ([1000, 30], 1), ([2, 2], 4), ([17, 4096], 1), ([4, 8, 10008], 1),
+            ([2048, 256], 2), ([2048, 2048], 1)]
 
 
 def hermitian(S, lengths):
