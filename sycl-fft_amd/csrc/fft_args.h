@@ -73,8 +73,11 @@ enum sfh_flags : uint32_t
     SFH_LOAD_LDS = 1u << 5,  // load through a coalescing LDS loop instead of into pass 0
     SFH_STORE_LDS = 1u << 6, // store through a coalescing LDS loop instead of from the last pass
     SFH_TILE_REVERSE = 1u << 7, // workgroup g runs tile ntiles - 1 - g (last tiles first)
-    // column-tile length-specialized launches: workgroups g and g + 8 (one XCD, as workgroups
-    // are dealt to the 8 XCDs in turn) run adjacent tiles, whose rows share cache lines
+    // column-tile length-specialized launches, bits 8-10: lg = log2 of the group of adjacent
+    // tiles (whose rows share cache lines) run on one XCD, as workgroups g, g + 8, g + 16, ...
+    // (workgroups are dealt to the 8 XCDs in turn); 0: no grouping, 1: pairs
+    SFH_TILE_XCD_SHIFT = 8,
+    SFH_TILE_XCD_MASK = 7u << 8,
     SFH_TILE_XCD_PAIR = 1u << 8
 };
 

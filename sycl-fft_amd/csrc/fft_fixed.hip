@@ -11,7 +11,7 @@ namespace
 using sfh::fx::radices;
 
 template <typename T, int NT, int WAVES, int MODE, bool PAD, bool SPLIT, bool PF, class R1, class R2, int W = 1>
-sfh_fixed_kernel entry(int max_tiles_per_cu = 0, int half_only = 0)
+sfh_fixed_kernel entry(int max_tiles_per_cu = 0, int half_only = 0, int xcd_lg = 0)
 {
     using prog = sfh::fx::program<R1, R2, MODE, PAD, W>;
     sfh_fixed_kernel k{};
@@ -32,6 +32,7 @@ sfh_fixed_kernel entry(int max_tiles_per_cu = 0, int half_only = 0)
                         : static_cast<uint32_t>(prog::slots() * elem * ((MODE & SFH_M_DBUF) ? 2 : 1));
     k.max_tiles_per_cu = max_tiles_per_cu;
     k.half_only = half_only;
+    k.xcd_lg = xcd_lg;
     k.n1 = R1::n;
     for (int i = 0; i < R1::n; ++i) {
         k.r1[i] = R1::v[i];
@@ -130,12 +131,19 @@ const sfh_fixed_kernel *table(int *count)
         entry<float, 128, 4, SFH_M_R2C, true, false, false, radices<32, 16, 8>, none>(),
         entry<float, 256, 2, SFH_M_R2C, true, false, false, radices<32, 16, 16>, none>(),
         entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<16, 16, 4>, none>(),
-        // column tiles (strided axes): complex<double> 4096 x 4 columns, 1024 work-items x 16
-        // points (256 per column), split regions of 33 KiB per column
-        entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(),
-        // complex<float> 2048 x 8 columns (rows of 64 bytes, as above), 1024 work-items x 16
-        // points, split regions of 8.3 KiB per column
-        entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 8>, none, 8>(),
+        // column tiles (strided axes), with the group of adjacent tiles run on one XCD (rows
+        // of 64 bytes share cache lines).  Groups of 1 (no grouping), 2, 4, 8 and 16 tiles,
+        // interleaved over two rounds: C3 complex<double> 4096^2 in place 228-257 / 199-229 /
+        // 198-199 / 216-219 / 203-205 us; complex<float> 2048^2 x 16 in place 562-564 / 543-546
+        // / 541 / 517-518 / 521 us; real 2048^2 x 16 R2C 356-364 / 325-328 / 311-313 / 307-311
+        // / 307 us, C2R 409-411 / 376-377 / 351-352 / 341 / 333-334 us (1025 half-spectrum
+        // columns: unaligned rows)
+        // complex<double> 4096 x 4 columns, 1024 work-items x 16 points (256 per column), split
+        // regions of 33 KiB per column, groups of 4 tiles
+        entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(0, 0, 2),
+        // complex<float> 2048 x 8 columns, 1024 work-items x 16 points, split regions of 8.3 KiB
+        // per column, groups of 8 tiles
+        entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 8>, none, 8>(0, 0, 3),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
     return k;

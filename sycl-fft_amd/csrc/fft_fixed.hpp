@@ -748,11 +748,13 @@ void fixed_fft(const sfh_tile_args a_byval)
     tile_ctx<T> c;
     uint64_t g = blockIdx.x;
     if constexpr (W > 1) {
-        // g = 8 r + x runs tile 16 (r / 2) + 2 x + (r mod 2): a bijection of the first
-        // 16-multiple of tiles that gives g and g + 8 the tile pair 2 m, 2 m + 1
-        if ((a.flags & SFH_TILE_XCD_PAIR) && g < (a.ntiles & ~uint64_t(15))) {
+        // groups of G = 2^lg tiles: g = 8 r + x runs tile G (8 (r / G) + x) + (r mod G), a
+        // bijection of the first 8G-multiple of tiles that gives g, g + 8, ..., g + 8 (G - 1)
+        // the adjacent tiles G m ... G m + G - 1
+        const uint32_t lg = (a.flags & SFH_TILE_XCD_MASK) >> SFH_TILE_XCD_SHIFT;
+        if (lg != 0 && g < (a.ntiles & ~((uint64_t(8) << lg) - 1))) {
             const uint64_t x = g & 7u, r = g >> 3;
-            g = (((r >> 1) << 3) + x) * 2 + (r & 1u);
+            g = ((((r >> lg) << 3) + x) << lg) + (r & ((uint64_t(1) << lg) - 1));
         }
     }
     set_tile<T, MODE>(a, c, g, threadIdx.x);

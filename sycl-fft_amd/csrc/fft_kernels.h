@@ -65,6 +65,8 @@ struct sfh_fixed_kernel
     // 1: used only for the half-length FFT of a real plan (where its C2R / R2C forms fuse
     // the tangle or untangle pass), not for complex plans of its length
     int half_only;
+    // column tiles: log2 of the group of adjacent tiles run on one XCD (SFH_TILE_XCD_SHIFT)
+    int xcd_lg;
 };
 extern "C" const sfh_fixed_kernel *sfh_fixed_kernels(int *count);
 extern "C" int sfh_launch_fixed(const sfh_fixed_kernel *kernel, const sfh_tile_args *args, hipStream_t stream);

@@ -955,10 +955,15 @@ static launch make_fixed_cols_launch(plan &p, const sfh_fixed_kernel &k, geometr
     a.out_sx = 1.0;
     a.out_sy = 1.0;
     a.flags |= SFH_WFAST;
-    // adjacent tiles on one XCD (SYCLFFT_HIP_XCD_PAIR=0: off, for measurements)
+    // groups of 2^xcd_lg adjacent tiles on one XCD (SYCLFFT_HIP_XCD_PAIR=0: off;
+    // SYCLFFT_HIP_XCD_GROUP=<lg>: groups of 2^lg tiles, lg = 1 .. 4, for measurements)
     const char *xp = std::getenv("SYCLFFT_HIP_XCD_PAIR");
     if (!(xp && std::strcmp(xp, "0") == 0)) {
-        a.flags |= SFH_TILE_XCD_PAIR;
+        uint32_t lg = static_cast<uint32_t>(std::min(4, std::max(0, k.xcd_lg)));
+        if (const char *xg = std::getenv("SYCLFFT_HIP_XCD_GROUP")) {
+            lg = static_cast<uint32_t>(std::min(4, std::max(1, std::atoi(xg))));
+        }
+        a.flags |= lg << SFH_TILE_XCD_SHIFT;
     }
     l.unit_dim = g.unit_dim;
     if (g.unit_dim == SFH_UNIT_BLO) {
