@@ -662,6 +662,14 @@ __device__ __forceinline__ float wave_xor_f(float v, int mask)
 {
     return __shfl_xor(v, mask);
 }
+__device__ __forceinline__ float wave_xor(float v, int mask)
+{
+    return __shfl_xor(v, mask);
+}
+__device__ __forceinline__ double wave_xor(double v, int mask)
+{
+    return __shfl_xor(v, mask);
+}
 __device__ __forceinline__ int exponent_of(float v)
 {
     return __builtin_amdgcn_frexp_expf(v);
@@ -711,6 +719,14 @@ inline void mfma16(sfh_f4 &acc, const uint32_t *a, const uint32_t *b)
 inline float wave_xor_f(float v, int mask)
 {
     return emu_wave_xor(v, mask);
+}
+inline float wave_xor(float v, int mask)
+{
+    return emu_wave_xor(v, mask);
+}
+inline double wave_xor(double v, int mask)
+{
+    return emu_wave_xor_d(v, mask);
 }
 inline int exponent_of(float v)
 {

@@ -115,6 +115,18 @@ float emu_wave_xor(float v, int mask)
     return r;
 }
 
+double emu_wave_xor_d(double v, int mask)
+{
+    static double buf[1024];
+    const unsigned t = threadIdx.x;
+    buf[t] = v;
+    emu_syncthreads();
+    const unsigned lane = t & 63u;
+    const double r = buf[(t - lane) + (lane ^ static_cast<unsigned>(mask))];
+    emu_syncthreads();
+    return r;
+}
+
 void emu_mfma_16x16x32(int prec, float *acc, const uint32_t *a, const uint32_t *b)
 {
     const unsigned t = threadIdx.x;

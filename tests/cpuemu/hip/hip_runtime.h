@@ -38,6 +38,7 @@ float emu_from_bf16(uint32_t h);
 uint32_t emu_to_f16(float v);   // round to nearest even (subnormals, overflow to inf)
 float emu_from_f16(uint32_t h);
 float emu_wave_xor(float v, int mask); // value of lane (lane ^ mask) of this wave of 64
+double emu_wave_xor_d(double v, int mask);
 // acc[j] += sum_k A[4(l/16)+j][k] B[k][l%16], the v_mfma_f32_16x16x32 lane layout
 void emu_mfma_16x16x32(int prec, float *acc, const uint32_t *a, const uint32_t *b);
 
