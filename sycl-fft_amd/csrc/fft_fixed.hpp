@@ -338,11 +338,10 @@ __device__ __forceinline__ void load_twiddles(const cx<T> *tw0, twregs<T, NT, L,
 // C2R tangle of one point: Z[j] = (X[j] + conj X[L-j]) + i W^-j (X[j] - conj X[L-j]),
 // W = W_2L, pre[j] = W^j, conjugated for a backward plan.
 template <typename T>
-__device__ __forceinline__ cx<T> tangle(const sfh_tile_args &a, const cx<T> &x, cx<T> m, uint32_t j)
+__device__ __forceinline__ cx<T> tangle(const sfh_tile_args &a, const cx<T> &x, cx<T> m, cx<T> w)
 {
     m.y = -m.y;
     const cx<T> s = x + m, d = x - m;
-    cx<T> w = static_cast<const cx<T> *>(a.pre)[j];
     w.y = -w.y;
     const cx<T> t = cmul(w, d);
     cx<T> z = {s.x - t.y, s.y + t.x};
@@ -369,6 +368,10 @@ __device__ __forceinline__ void load_c2r_paired(const sfh_tile_args &a, const ti
     if (i == 0u) {
         xl = src[L];
     }
+    // W^(i + t M) = W^i W^(t M): one table load per butterfly, the W^(t M) the same for every
+    // lane (scalar loads)
+    const cx<T> *pre = static_cast<const cx<T> *>(a.pre);
+    const cx<T> wi = pre[i];
     const bool half = i == static_cast<uint32_t>(M / 2);
 #pragma unroll
     for (int p = 0; p < R / 2; ++p) {
@@ -379,21 +382,21 @@ __device__ __forceinline__ void load_c2r_paired(const sfh_tile_args &a, const ti
             m0 = x1;
             m1 = x0;
         }
-        const cx<T> z0 = tangle(a, x0, m0, i + static_cast<uint32_t>(p * M));
-        const cx<T> z1 = tangle(a, x1, m1, i + static_cast<uint32_t>(u * M));
+        const cx<T> z0 = tangle(a, x0, m0, cmul(wi, pre[p * M]));
+        const cx<T> z1 = tangle(a, x1, m1, cmul(wi, pre[u * M]));
         if (i != 0u) {
             v[p] = z0;
             v[u] = z1;
         }
     }
     if (i == 0u) {
-        const cx<T> z0 = tangle(a, v[0], xl, 0u);
-        const cx<T> zh = tangle(a, v[R / 2], v[R / 2], static_cast<uint32_t>((R / 2) * M));
+        const cx<T> z0 = tangle(a, v[0], xl, pre[0]);
+        const cx<T> zh = tangle(a, v[R / 2], v[R / 2], pre[(R / 2) * M]);
 #pragma unroll
         for (int t = 1; t < R / 2; ++t) {
             const cx<T> x0 = v[t], x1 = v[R - t];
-            v[t] = tangle(a, x0, x1, static_cast<uint32_t>(t * M));
-            v[R - t] = tangle(a, x1, x0, static_cast<uint32_t>((R - t) * M));
+            v[t] = tangle(a, x0, x1, pre[t * M]);
+            v[R - t] = tangle(a, x1, x0, pre[(R - t) * M]);
         }
         v[0] = z0;
         v[R / 2] = zh;
@@ -635,15 +638,22 @@ __device__ __forceinline__ void store_r2c(const sfh_tile_args &a, const tile_ctx
     }
     __syncthreads();
     const epilogue<T> ep(a);
+    // W^(j + u NT) = W^j W^(u NT): one table load per work-item, the W^(u NT) the same for
+    // every lane (scalar loads)
     const cx<T> *w = static_cast<const cx<T> *>(a.pre);
-    for (uint32_t k = tid; k <= static_cast<uint32_t>(L); k += static_cast<uint32_t>(NT)) {
-        const cx<T> zk = z[k == static_cast<uint32_t>(L) ? 0u : k];
-        cx<T> zm = z[k == 0u ? 0u : static_cast<uint32_t>(L) - k];
-        zm.y = -zm.y;
-        const cx<T> s = zk + zm, d = zk - zm;
-        const cx<T> t = cmul(w[k], d);
-        // 1/2 (s - i t) = 1/2 (s.x + t.y, s.y - t.x)
-        out_global(c.dst)[k] = ep(cx<T>{T(0.5) * (s.x + t.y), T(0.5) * (s.y - t.x)});
+    const cx<T> wj = w[tid];
+#pragma unroll
+    for (int u = 0; u <= L / NT; ++u) {
+        const uint32_t k = tid + static_cast<uint32_t>(u * NT);
+        if (k <= static_cast<uint32_t>(L)) {
+            const cx<T> zk = z[k == static_cast<uint32_t>(L) ? 0u : k];
+            cx<T> zm = z[k == 0u ? 0u : static_cast<uint32_t>(L) - k];
+            zm.y = -zm.y;
+            const cx<T> s = zk + zm, d = zk - zm;
+            const cx<T> t = cmul(cmul(wj, w[u * NT]), d);
+            // 1/2 (s - i t) = 1/2 (s.x + t.y, s.y - t.x)
+            out_global(c.dst)[k] = ep(cx<T>{T(0.5) * (s.x + t.y), T(0.5) * (s.y - t.x)});
+        }
     }
 }
 
