@@ -148,7 +148,8 @@ struct program
     // (the tangle partners: point L - (i + t M) is point R - 1 - t of butterfly M - i): the
     // first step of C2R programs.  (The R2C untangle after the last step goes through LDS:
     // the same pairing there, with the mirror outputs exchanged between lanes, measured
-    // 0.163 / 0.138 ms against 0.149 / 0.125 ms for fp32 4096 x 16384 / f64 4096 x 8192.)
+    // 0.155-0.159 / 0.135-0.137 ms against 0.147-0.151 / 0.123-0.124 ms for fp32 4096 x 16384
+    // / f64 4096 x 8192, both with the factored twiddles.)
     static constexpr bool paired(int s)
     {
         return (MODE & SFH_M_C2R) != 0 && s == 0;
