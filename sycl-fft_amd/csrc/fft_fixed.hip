@@ -126,14 +126,14 @@ const sfh_fixed_kernel *table(int *count)
         entry<float, 128, 4, SFH_M_C2R, true, true, false, radices<32, 16, 8>, none>(),
         entry<float, 256, 2, SFH_M_C2R, true, true, false, radices<32, 16, 16>, none>(),
         entry<float, 64, 4, SFH_M_C2R, true, true, false, radices<16, 16, 4>, none>(),
-        // R2C real plans: the untangle fused into the last pass (one more LDS exchange, whole
-        // complex values), same radix programs as the plain entries
-        entry<double, 128, 4, SFH_M_R2C, true, false, false, radices<16, 16, 8>, none>(),
-        entry<double, 256, 4, SFH_M_R2C, true, false, false, radices<16, 16, 16>, none>(),
-        entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<32, 16, 4>, none>(),
-        entry<float, 128, 4, SFH_M_R2C, true, false, false, radices<32, 16, 8>, none>(),
-        entry<float, 256, 2, SFH_M_R2C, true, false, false, radices<32, 16, 16>, none>(),
-        entry<float, 64, 4, SFH_M_R2C, true, false, false, radices<16, 16, 4>, none>(),
+        // R2C real plans: the untangle fused into the last pass (one more split LDS exchange),
+        // same radix programs as the plain entries
+        entry<double, 128, 4, SFH_M_R2C, true, true, false, radices<16, 16, 8>, none>(),
+        entry<double, 256, 4, SFH_M_R2C, true, true, false, radices<16, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_R2C, true, true, false, radices<32, 16, 4>, none>(),
+        entry<float, 128, 4, SFH_M_R2C, true, true, false, radices<32, 16, 8>, none>(),
+        entry<float, 256, 2, SFH_M_R2C, true, true, false, radices<32, 16, 16>, none>(),
+        entry<float, 64, 4, SFH_M_R2C, true, true, false, radices<16, 16, 4>, none>(),
         // column tiles (strided axes), with the group of adjacent tiles run on one XCD (rows
         // of 64 bytes share cache lines).  Groups of 1 (no grouping), 2, 4, 8 and 16 tiles,
         // interleaved over two rounds: C3 complex<double> 4096^2 in place 228-257 / 199-229 /

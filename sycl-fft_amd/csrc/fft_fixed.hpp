@@ -149,7 +149,7 @@ struct program
     // first step of C2R programs.  (The R2C untangle after the last step goes through LDS:
     // the same pairing there, with the mirror outputs exchanged between lanes, measured
     // 0.155-0.159 / 0.135-0.137 ms against 0.147-0.151 / 0.123-0.124 ms for fp32 4096 x 16384
-    // / f64 4096 x 8192, both with the factored twiddles.)
+    // / f64 4096 x 8192 through LDS with whole-value exchanges.)
     static constexpr bool paired(int s)
     {
         return (MODE & SFH_M_C2R) != 0 && s == 0;
@@ -613,47 +613,71 @@ __device__ __forceinline__ void store(const sfh_tile_args &a, const tile_ctx<T> 
     }
 }
 
-// R2C untangle (SFH_M_R2C): the last step's outputs Z[k] go to LDS in natural order (whole
-// complex values: R2C kernels exchange without the split), then work-item j computes outputs
-// k = j, j + NT, ... <= L from Z[k] and Z[L - k]:
-//   X[k] = 1/2 (Z[k] + conj Z[L-k]) - i/2 W^k (Z[k] - conj Z[L-k]),  W = W_2L, Z[L] = Z[0],
-// and stores them through the epilogue (scale) to the tile's destination (L + 1 points).
+// R2C untangle (SFH_M_R2C): after the last step, work-item j takes the output pairs k, L - k
+// for k = j, j + NT, ... <= L / 2, which both come from Z[k] and Z[L - k] (Z[L] = Z[0]):
+//   X[k]     = 1/2 (Z[k] + conj Z[L-k]) - i/2 W^k (Z[k] - conj Z[L-k]),  W = W_2L,
+//   X[L - k] = the same with Z[k] and Z[L-k] swapped and W^(L-k) = -conj W^k.
+// Z goes through the LDS plane of the split exchange in natural order, real parts first
+// (each work-item keeps Re Z[k], Re Z[L-k] of its pairs), then imaginary parts; the
+// outputs (L + 1 per transform) go through the epilogue (scale).  W^(j + u NT) = W^j
+// W^(u NT): one table load per work-item, W^(u NT) uniform over the lanes.
 template <typename T, int NT, int L, int MODE, class PROG, int S>
 __device__ __forceinline__ void store_r2c(const sfh_tile_args &a, const tile_ctx<T> &c, unsigned char *lds,
                                           const regs<T, NT, L, PROG, S> &r)
 {
     using sh = shape<NT, L, PROG, S>;
     constexpr int R = sh::R, M = sh::M;
+    constexpr int NP = (L / 2 + NT) / NT; // pairs per work-item (k <= L / 2)
     const uint32_t tid = lane<PROG>();
-    cx<T> *z = reinterpret_cast<cx<T> *>(lds);
+    T *z = reinterpret_cast<T *>(lds);
+    T re[NP][2], im[NP][2];
     __syncthreads(); // the last exchange's LDS reads are done
 #pragma unroll
-    for (int q = 0; q < sh::NB; ++q) {
-        const uint32_t i = tid + static_cast<uint32_t>(q * NT);
-        if (sh::has(q, i)) {
+    for (int part = 0; part < 2; ++part) {
 #pragma unroll
-            for (int t = 0; t < R; ++t) {
-                z[i + static_cast<uint32_t>(t * M)] = r.v[q][t];
+        for (int q = 0; q < sh::NB; ++q) {
+            const uint32_t i = bfly<NT, L, PROG, S>(tid, q);
+            if (sh::has(q, i)) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    z[i + static_cast<uint32_t>(t * M)] = part == 0 ? r.v[q][t].x : r.v[q][t].y;
+                }
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NP; ++u) {
+            const uint32_t k = tid + static_cast<uint32_t>(u * NT);
+            if (k <= static_cast<uint32_t>(L / 2)) {
+                T *dst = part == 0 ? re[u] : im[u];
+                dst[0] = z[k];
+                dst[1] = z[k == 0u ? 0u : static_cast<uint32_t>(L) - k];
+            }
+        }
+        if (part == 0) {
+            __syncthreads(); // the real parts are read before the imaginary parts overwrite them
+        }
     }
-    __syncthreads();
     const epilogue<T> ep(a);
-    // W^(j + u NT) = W^j W^(u NT): one table load per work-item, the W^(u NT) the same for
-    // every lane (scalar loads)
     const cx<T> *w = static_cast<const cx<T> *>(a.pre);
     const cx<T> wj = w[tid];
+    const auto untangle = [&](const cx<T> &zk, cx<T> zm, const cx<T> &wk, uint32_t k) {
+        zm.y = -zm.y;
+        const cx<T> s = zk + zm, d = zk - zm;
+        const cx<T> t = cmul(wk, d);
+        // 1/2 (s - i t) = 1/2 (s.x + t.y, s.y - t.x)
+        out_global(c.dst)[k] = ep(cx<T>{T(0.5) * (s.x + t.y), T(0.5) * (s.y - t.x)});
+    };
 #pragma unroll
-    for (int u = 0; u <= L / NT; ++u) {
+    for (int u = 0; u < NP; ++u) {
         const uint32_t k = tid + static_cast<uint32_t>(u * NT);
-        if (k <= static_cast<uint32_t>(L)) {
-            const cx<T> zk = z[k == static_cast<uint32_t>(L) ? 0u : k];
-            cx<T> zm = z[k == 0u ? 0u : static_cast<uint32_t>(L) - k];
-            zm.y = -zm.y;
-            const cx<T> s = zk + zm, d = zk - zm;
-            const cx<T> t = cmul(cmul(wj, w[u * NT]), d);
-            // 1/2 (s - i t) = 1/2 (s.x + t.y, s.y - t.x)
-            out_global(c.dst)[k] = ep(cx<T>{T(0.5) * (s.x + t.y), T(0.5) * (s.y - t.x)});
+        if (k <= static_cast<uint32_t>(L / 2)) {
+            const cx<T> zk = {re[u][0], im[u][0]}, zm = {re[u][1], im[u][1]};
+            const cx<T> wk = cmul(wj, w[u * NT]);
+            untangle(zk, zm, wk, k);
+            if (k < static_cast<uint32_t>(L / 2)) {
+                untangle(zm, zk, cx<T>{-wk.x, wk.y}, static_cast<uint32_t>(L) - k);
+            }
         }
     }
 }
@@ -832,8 +856,8 @@ void fixed_fft(const sfh_tile_args a_byval)
     static_assert(!(MODE & SFH_M_T4), "no four-step twiddle in the length-specialized kernels");
     static_assert(!(MODE & SFH_M_C2R) || (W == 1 && (MODE & ~(SFH_M_C2R | SFH_M_DBUF)) == 0),
                   "C2R tangle: one contiguous transform per tile, plain mode");
-    static_assert(!(MODE & SFH_M_R2C) || (W == 1 && !SPLIT && (MODE & ~(SFH_M_R2C | SFH_M_DBUF)) == 0),
-                  "R2C untangle: one contiguous transform per tile, whole-value exchanges, plain mode");
+    static_assert(!(MODE & SFH_M_R2C) || (W == 1 && SPLIT && (MODE & ~SFH_M_R2C) == 0),
+                  "R2C untangle: one contiguous transform per tile, split exchanges, plain mode");
     static_assert(!(MODE & SFH_M_MID) || (fx::length_of<R2>() == L && R2::v[0] == R1::v[R1::n - 1]),
                   "second FFT: same length, first radix = the first FFT's last");
 #if defined(__HIP_DEVICE_COMPILE__)
