@@ -145,7 +145,10 @@ const sfh_fixed_kernel *table(int *count)
         // regions of 33 KiB per column, groups of 4 tiles
         entry<double, 1024, 4, 0, true, true, false, radices<16, 16, 16>, none, 4>(0, 0, 2),
         // complex<float> 2048 x 8 columns, 1024 work-items x 16 points, split regions of 8.3 KiB
-        // per column, groups of 8 tiles
+        // per column, groups of 8 tiles.  16 columns per tile (rows of 128 bytes, 1024 x 32
+        // points, one tile per CU, groups of 4) measured slower: real 2-D 2048^2 x 16 R2C
+        // 316-323 against 291-293 us, C2R 351-355 against 309-315 us, fp32 2048^2 x 16 526
+        // against 517-518 us
         entry<float, 1024, 4, 0, true, true, false, radices<16, 16, 8>, none, 8>(0, 0, 3),
     };
     *count = static_cast<int>(sizeof(k) / sizeof(k[0]));
