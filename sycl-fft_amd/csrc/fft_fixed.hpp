@@ -424,15 +424,19 @@ __device__ __forceinline__ void compute(const sfh_tile_args &a, const tile_ctx<T
                 for (int t = 0; t < R; ++t) {
                     v[t] = load_point<T, MODE, PROG::cols>(a, c, i + static_cast<uint32_t>(t * M));
                 }
-            } else if constexpr (NS > 1 && !PF && sizeof(T) == 4 && R >= 5) {
-                // complex<float>, radix >= 5: load W^(j 2^b) (rows 2^b - 1 of the pass table)
+            } else if constexpr (NS > 1 && !PF && (sizeof(T) == 4 || R % 2 == 1) && R >= 5) {
+                // complex<float> radix >= 5, complex<double> odd radix >= 5: load W^(j 2^b)
+                // (rows 2^b - 1 of the pass table)
                 // and form W^(j t) = W^(j h) W^(j (t - h)), h the highest power of two in t:
                 // floor(log2 (R - 1)) + 1 loads per butterfly instead of R - 1, at most
                 // log2 R - 1 products deep.  At 512 MiB batches, against one load per twiddle
                 // (power-of-two radices): 4096 206-209 against 217-219 us, 8192 205 / 216-219,
                 // 16384 229-231 / 244-245, 32768 280-282 / 296-297, the Bluestein pair (10007 x
                 // 128) 28.2-28.3 / 29.7-29.9 us; its radix-5 passes too: 28.3-28.4 / 28.8-28.9 us.
-                // complex<double> ran slower (4096: 210 against 196 us) and keeps the table loads
+                // complex<double> takes it for the odd radices (6561 x 512 26.3-26.7 against
+                // 27.1-27.4 us, 2520 x 1024 20.0-20.1 against 20.3-20.9 us); with radix 16 the
+                // products made it slower (4096: 210 against 196 us), so its power-of-two passes
+                // keep one table load per twiddle
                 const uint32_t j = i % static_cast<uint32_t>(NS);
                 const cx<T> *tw = tw0 + st.twoff;
                 cx<T> wp[R];
