@@ -92,10 +92,13 @@ const sfh_fixed_kernel *table(int *count)
         // 8192: 256 work-items x 32 points, 32 16 16 (two exchanges, pad-linear): 209-210 us
         // against 214 us for 16 2 16 16
         entry<float, 256, 2, 0, true, true, false, radices<32, 16, 16>, none>(),
-        // 16384: 512 work-items x 32 points (two tiles per CU, 106 VGPRs).  Measured at 512 MiB
-        // against 256 x 64 (16 16 16 4, 223 VGPRs): 0.54 against 0.50-0.52 of the HBM peak; the
-        // radix orders 8 16 16 8 and 16 8 16 8 ran within 1%, twiddle prefetch spilled (0.44);
-        // with radix 32, 32 16 32 ran within 1% and 32 32 16 12% slower
+        // 16384: 512 work-items x 32 points (two tiles per CU).  Measured at 512 MiB against
+        // 256 x 64 (16 16 16 4, 223 VGPRs): 0.54 against 0.50-0.52 of the HBM peak; the radix
+        // orders 8 16 16 8 and 16 8 16 8 ran within 1%, twiddle prefetch spilled (0.44); with
+        // radix 32, 32 32 16 12% slower.  With the factored pass twiddles 32 16 32 (two
+        // exchanges; 8 VGPRs spilled) runs 225.0-230.3 us against 228.6 us for 16 8 8 16 (three
+        // exchanges, next entry) and 226.9-229.1 us for 16 32 32
+        entry<float, 512, 4, 0, true, true, false, radices<32, 16, 32>, none>(),
         entry<float, 512, 4, 0, true, true, false, radices<16, 8, 8, 16>, none>(),
         // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
         // the 64 points of each work-item stay in registers across every exchange; 32 32 32 (two
