@@ -102,7 +102,9 @@ MANY = ["f:4096:3:-1:0:0", "f:300:5:1:3:0", "d:2520:3:1:3:1", "f:10007:3:-1:3:0"
 COLS = ["d:4096x4:2:-1:0:0", "d:4096x8:1:1:3:1", "d:4096x12:1:-1:1:0", "d:4096x2x4:1:1:2:0", "d:4096x4:3:1:3:0",
         # ragged last tiles (33 and 65 columns), fp32 2048 x 8 columns, multi-D real plans over them
         "d:4096x33:1:-1:0:0", "c:2048x65:1:-1:1:0", "c:2048x8:2:1:3:1", "R:2048x128:2:-1:0:0",
-        "R:2048x128:2:1:2:0"]
+        "R:2048x128:2:1:2:0",
+        # 64 and 32 tiles: every tile goes through the XCD group remap (8 and 4 tiles per group)
+        "c:2048x512:1:-1:0:0", "d:4096x128:1:1:3:0"]
 
 
 def test_emulated_column_tiles():
