@@ -103,7 +103,7 @@ const sfh_fixed_kernel *table(int *count)
         // 32768 in one pass: with the split exchange one plane (128 KiB + pad) fits the LDS, and
         // the 64 points of each work-item stay in registers across every exchange; 32 32 32 (two
         // exchanges) measured 302 us against 306-307 us for 16 16 16 8, and 1024 x 32 points
-        // 303-307 us
+        // 303-307 us (with the factored pass twiddles: 279.5-286.5 against 280.2-281.5 us)
         entry<float, 512, 2, 0, true, true, false, radices<32, 32, 32>, none>(),
         // complex<double> 4096 and 2048: 16 points per work-item, split tiles (33 and 16.5 KiB).
         // At 512 MiB batches: 4096 196.5-196.9 us (0.68 of the HBM peak) against 218-219 us on
